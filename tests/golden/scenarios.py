@@ -1,0 +1,334 @@
+"""Declarative FDTD scenarios shared by the fixture generator, the oracle tests
+and the HIP parity tests.
+
+A scenario is a plain dict:
+
+    shape, grid_spacing, courant_number, permittivity, permeability
+    init        : optional {"E": array, "H": array} written before stepping
+    components  : list of (kind, params, key, overrides) in registration order
+                  kind      - class name in the fdtd API ("PML", "Object", ...)
+                  params    - constructor keyword arguments
+                  key       - the index passed to ``grid[key] = component``
+                  overrides - attributes assigned on the component right after
+                              registration (e.g. a full 3x3 inverse
+                              permittivity for an AnisotropicObject)
+    program     : list of ops: ("run", T) | ("step", T) | ("reset",) |
+                  ("manual_pulse", T, (x, y, z, comp))
+
+``build_fdtd(mod, sc)`` realises a scenario with any module exposing the fdtd
+API (the reference package ``fdtd`` when generating fixtures, ``fdtd_amd`` in
+the parity tests). ``run_program`` executes the op list. All random inputs come
+from ``numpy.random.RandomState`` (a stream numpy keeps stable across
+releases); fixtures also store a checksum of the inputs to detect drift.
+"""
+
+import math
+
+import numpy as np
+
+C0 = 299792458.0  # speed of light (m/s), fdtd/constants.py:5
+
+
+def _spd_inverse_field(rs, shape, lo=1.0, hi=9.0):
+    """Per-cell inverse of a random SPD 3x3 permittivity with eigenvalues in [lo, hi]."""
+    n = int(np.prod(shape))
+    out = np.empty((n, 3, 3))
+    for i in range(n):
+        q, _ = np.linalg.qr(rs.randn(3, 3))
+        lam = rs.uniform(lo, hi, size=3)
+        eps = q.T @ np.diag(lam) @ q
+        out[i] = np.linalg.inv(eps)
+    return out.reshape(tuple(shape) + (3, 3))
+
+
+def scenario_c1():
+    """The 2D plain-method configuration (161x97x1, PML on x/y, periodic z,
+    continuous 1550 nm LineSource, LineDetector at x=140, eps=1.7^2 block)."""
+    s = slice
+    return dict(
+        name="c1",
+        shape=(161, 97, 1),
+        grid_spacing=155e-9,
+        courant_number=2 ** -0.5,
+        permittivity=1.0,
+        permeability=1.0,
+        components=[
+            ("PML", dict(name="pml_xlow"), (s(0, 10), s(None), s(None)), {}),
+            ("PML", dict(name="pml_xhigh"), (s(-10, None), s(None), s(None)), {}),
+            ("PML", dict(name="pml_ylow"), (s(None), s(0, 10), s(None)), {}),
+            ("PML", dict(name="pml_yhigh"), (s(None), s(-10, None), s(None)), {}),
+            ("PeriodicBoundary", dict(name="zbounds"), (s(None), s(None), 0), {}),
+            ("LineSource", dict(period=1550e-9 / C0, name="source"), (20, s(20, 77), 0), {}),
+            ("LineDetector", dict(name="detector"), (140, s(None), 0), {}),
+            ("Object", dict(permittivity=1.7 ** 2, name="object"), (s(11, 32), s(30, 84), 0), {}),
+        ],
+        program=[("run", 300)],
+    )
+
+
+def scenario_g2():
+    """Small 3D run touching every hot-path feature: six PML faces of unequal
+    thickness, per-voxel eps (3D) and per-component mu (4D), overlapping Object
+    and AbsorbingObject, an AnisotropicObject whose inverse permittivity is
+    overwritten by a full SPD inverse, pulsed/continuous point and line sources
+    (one with a float period in seconds, one diagonal), a hard PlaneSource, two
+    LineDetectors (one diagonal) and a BlockDetector, random initial fields."""
+    rs = np.random.RandomState(1234)
+    shape = (24, 20, 16)
+    eps = rs.uniform(1.0, 4.0, size=shape)
+    mu = rs.uniform(1.0, 2.0, size=shape + (3,))
+    e0 = 0.05 * rs.randn(*shape, 3)
+    h0 = 0.05 * rs.randn(*shape, 3)
+    aniso_eps = rs.uniform(1.5, 3.0, size=(6, 5, 4, 3))
+    aniso_inv = _spd_inverse_field(rs, (6, 5, 4))
+    s = slice
+    return dict(
+        name="g2",
+        shape=shape,
+        grid_spacing=100e-9,
+        courant_number=0.5,
+        permittivity=eps,
+        permeability=mu,
+        init=dict(E=e0, H=h0),
+        components=[
+            ("PML", dict(), (s(0, 4), s(None), s(None)), {}),
+            ("PML", dict(a=2e-8), (s(None), s(-5, None), s(None)), {}),
+            ("PML", dict(), (s(-4, None), s(None), s(None)), {}),
+            ("PML", dict(), (s(None), s(0, 3), s(None)), {}),
+            ("PML", dict(), (s(None), s(None), s(0, 4)), {}),
+            ("PML", dict(), (s(None), s(None), s(-4, None)), {}),
+            ("Object", dict(permittivity=2.5, name="obj"), (s(6, 12), s(5, 11), s(4, 9)), {}),
+            ("AbsorbingObject", dict(permittivity=1.5, conductivity=2e4, name="absorber"),
+             (s(10, 16), s(8, 14), s(6, 12)), {}),
+            ("AnisotropicObject", dict(permittivity=aniso_eps, name="aniso"),
+             (s(14, 20), s(3, 8), s(9, 13)), {"inverse_permittivity": aniso_inv}),
+            ("PointSource", dict(period=10, pulse=True, name="psrc"), (12, 10, 8), {}),
+            ("LineSource", dict(period=1.3e-14, phase_shift=0.3, amplitude=0.7, name="lsrc"),
+             (s(4, 18), 15, 10), {}),
+            ("LineSource", dict(period=12, pulse=True, cycle=3, hanning_dt=4.0, name="dsrc"),
+             (s(5, 15), s(2, 9), 13), {}),
+            ("PlaneSource", dict(period=15, amplitude=0.5, polarization="y", name="plane"),
+             (7, s(4, 16), s(3, 13)), {}),
+            ("LineDetector", dict(name="ld"), (s(2, 22), 10, 8), {}),
+            ("LineDetector", dict(name="ld_diag"), (s(3, 19), s(4, 16), s(2, 14)), {}),
+            ("BlockDetector", dict(name="bdet"), (s(10, 12), s(5, 7), s(3, 4)), {}),
+        ],
+        program=[("run", 60)],
+    )
+
+
+def _periodic_scenario(name, axis, periodic_first, steps=30):
+    """Periodic boundary on `axis` combined with PML faces on the two other
+    axes, registered either before or after the periodic boundary; the order
+    decides which plane-0 values the PML corrections see."""
+    rs = np.random.RandomState(77 + axis + (10 if periodic_first else 0))
+    shape = (12, 10, 8)
+    s = slice
+    full = [s(None)] * 3
+    per_key = list(full)
+    per_key[axis] = 0 if periodic_first else -1
+    per = ("PeriodicBoundary", dict(), tuple(per_key), {})
+    pmls = []
+    for other in range(3):
+        if other == axis:
+            continue
+        lo = list(full)
+        lo[other] = s(0, 3)
+        hi = list(full)
+        hi[other] = s(-2, None)
+        pmls.append(("PML", dict(), tuple(lo), {}))
+        pmls.append(("PML", dict(), tuple(hi), {}))
+    comps = [per] + pmls if periodic_first else pmls[:2] + [per] + pmls[2:]
+    src_key = [5, 4, 3]
+    comps.append(("PointSource", dict(period=9), tuple(src_key), {}))
+    det_key = [s(None)] * 3
+    det_key[(axis + 1) % 3] = 2
+    det_key[(axis + 2) % 3] = 1
+    comps.append(("LineDetector", dict(), tuple(det_key), {}))
+    return dict(
+        name=name,
+        shape=shape,
+        grid_spacing=80e-9,
+        courant_number=0.55,
+        permittivity=rs.uniform(1.0, 3.0, size=shape),
+        permeability=1.0,
+        init=dict(E=0.1 * rs.randn(*shape, 3), H=0.1 * rs.randn(*shape, 3)),
+        components=comps,
+        program=[("run", steps)],
+    )
+
+
+def scenario_manual():
+    """The manual half-stepping pattern of the reference detector test: the
+    host writes E between update_E and update_H and advances the counter by
+    hand; then reset() (which keeps the PML state) and a plain run."""
+    s = slice
+    return dict(
+        name="manual",
+        shape=(10, 10, 10),
+        grid_spacing=100e-9,
+        courant_number=None,
+        permittivity=1.0,
+        permeability=1.0,
+        components=[
+            ("PML", dict(), (s(0, 3), s(None), s(None)), {}),
+            ("PML", dict(), (s(None), s(None), s(-3, None)), {}),
+            ("AbsorbingObject", dict(permittivity=1.0, conductivity=1e9), (s(3, 5), s(3, 5), s(3, 5)), {}),
+            ("BlockDetector", dict(), (s(4, 4), s(4, 4), s(4, 4)), {}),
+            ("LineDetector", dict(), (s(1, 9), 6, 6), {}),
+        ],
+        program=[("manual_pulse", 40, (4, 4, 4, 2)), ("reset",), ("step", 12)],
+    )
+
+
+def scenario_float_keys():
+    """Registration with positions in metres (floats) and a float period:
+    exercises the distance/time rounding rules of the grid."""
+    s = slice
+    dx = 50e-9
+    return dict(
+        name="float_keys",
+        shape=(1.0e-6, 0.8e-6, 12),
+        grid_spacing=dx,
+        courant_number=0.5,
+        permittivity=1.0,
+        permeability=1.0,
+        components=[
+            ("PML", dict(), (s(0, 3.1e-7), s(None), s(None)), {}),
+            ("Object", dict(permittivity=3.0), (s(4.0e-7, 6.2e-7), s(2.1e-7, 5.0e-7), s(2, 9)), {}),
+            ("PointSource", dict(period=2.0e-15, phase_shift=1.1), (3.0e-7, 4.0e-7, 6), {}),
+            ("LineSource", dict(period=11), (s(2.0e-7, 8.0e-7), 5.0e-7, 5), {}),
+            ("LineDetector", dict(), (s(None), 7, 6), {}),
+        ],
+        program=[("run", 25)],
+    )
+
+
+def scenario_1d():
+    """1D grid (N,1,1): a hard-sourced pulse travels along x."""
+    return dict(
+        name="oned",
+        shape=(64, 1, 1),
+        grid_spacing=100e-9,
+        courant_number=1.0,
+        permittivity=1.0,
+        permeability=1.0,
+        components=[
+            ("PML", dict(), (slice(-8, None), slice(None), slice(None)), {}),
+            ("PointSource", dict(period=16, pulse=True, cycle=2, hanning_dt=8.0), (10, 0, 0), {}),
+            ("LineDetector", dict(), (slice(None), 0, 0), {}),
+        ],
+        program=[("run", 50)],
+    )
+
+
+def all_scenarios():
+    out = [scenario_c1(), scenario_g2(), scenario_manual(), scenario_float_keys(), scenario_1d()]
+    for axis in range(3):
+        for first in (True, False):
+            out.append(_periodic_scenario(f"periodic_{'xyz'[axis]}_{'first' if first else 'last'}", axis, first))
+    return out
+
+
+def get_scenario(name):
+    for sc in all_scenarios():
+        if sc["name"] == name:
+            return sc
+    raise KeyError(name)
+
+
+def input_checksum(sc):
+    """A float checksum over every array input of a scenario."""
+    acc = 0.0
+    arrays = []
+    for k in ("permittivity", "permeability"):
+        if isinstance(sc[k], np.ndarray):
+            arrays.append(sc[k])
+    for v in sc.get("init", {}).values():
+        arrays.append(v)
+    for _, params, _, overrides in sc["components"]:
+        for v in list(params.values()) + list(overrides.values()):
+            if isinstance(v, np.ndarray):
+                arrays.append(v)
+    for i, a in enumerate(arrays):
+        acc += float(np.sum(np.asarray(a, dtype=np.float64) * (i + 1)))
+    return acc
+
+
+# ---------------------------------------------------------------------------
+# realisation through an fdtd-compatible module
+
+
+def build_fdtd(mod, sc, to_array=None):
+    """Build a grid with the fdtd API of module `mod`. `to_array` converts a
+    numpy input to the module's array type (identity for numpy backends)."""
+    conv = to_array or (lambda a: a)
+    kw = dict(shape=sc["shape"], grid_spacing=sc["grid_spacing"])
+    if sc["courant_number"] is not None:
+        kw["courant_number"] = sc["courant_number"]
+    for k in ("permittivity", "permeability"):
+        v = sc[k]
+        kw[k] = conv(v) if isinstance(v, np.ndarray) else v
+    grid = mod.Grid(**kw)
+    comps = []
+    for kind, params, key, overrides in sc["components"]:
+        p = {k: (conv(v) if isinstance(v, np.ndarray) else v) for k, v in params.items()}
+        comp = getattr(mod, kind)(**p)
+        grid[key] = comp
+        for attr, val in overrides.items():
+            setattr(comp, attr, conv(val) if isinstance(val, np.ndarray) else val)
+        comps.append(comp)
+    init = sc.get("init")
+    if init:
+        grid.E[...] = conv(init["E"])
+        grid.H[...] = conv(init["H"])
+    return grid, comps
+
+
+def _gaussian(x, fwhm, center):
+    # normalized_gaussian_pulse, fdtd/waveforms.py:34-46 (peak 1 at `center`)
+    sigma = fwhm / (2.0 * math.sqrt(2.0 * math.log(2)))
+    return np.exp(-(((x - center) ** 2.0) / (2.0 * (sigma ** 2.0))))
+
+
+def run_program(grid, sc, write=None):
+    """Execute the scenario's op list on an fdtd-API grid. `write(grid, idx,
+    value)` performs a host write into grid.E (default: item assignment)."""
+    for op in sc["program"]:
+        if op[0] == "run":
+            grid.run(op[1], progress_bar=False)
+        elif op[0] == "step":
+            for _ in range(op[1]):
+                grid.step()
+        elif op[0] == "reset":
+            grid.reset()
+        elif op[0] == "manual_pulse":
+            n, idx = op[1], op[2]
+            for _ in range(n):
+                grid.update_E()
+                val = float(_gaussian(grid.time_passed, (n / 10.0) * grid.time_step, (n / 2) * grid.time_step))
+                if write is None:
+                    grid.E[idx] = val
+                else:
+                    write(grid, idx, val)
+                grid.update_H()
+                grid.time_steps_passed += 1
+        else:
+            raise ValueError(op)
+
+
+def build_oracle(oracle_mod, sc):
+    """Build the same scenario with the oracle's own registration API."""
+    sim = oracle_mod.Sim(sc["shape"], sc["grid_spacing"], sc["permittivity"], sc["permeability"],
+                         sc["courant_number"])
+    comps = []
+    for kind, params, key, overrides in sc["components"]:
+        comp = sim.add(kind, key, **params)
+        for attr, val in overrides.items():
+            setattr(comp, attr, val)
+        comps.append(comp)
+    init = sc.get("init")
+    if init:
+        sim.E[...] = init["E"]
+        sim.H[...] = init["H"]
+    return sim, comps
