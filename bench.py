@@ -1,0 +1,171 @@
+"""Throughput of the fused FDTD step on one MI355X.
+
+Workload (the 256^3 plain-method configuration): Grid 256x256x256, spacing
+50 nm, courant 1/sqrt(3), 24 seeded dielectric spheres (radius 8-32 cells,
+eps uniform in [1, 12]) rasterised into the grid permittivity, 16-cell PML on
+all six faces, pulsed 1550 nm PointSource at the centre. A "step" is one full
+Yee step (E then H half step) over the whole grid.
+
+    python bench.py [--steps K] [--warmup W] [--dtype float64|float32]
+
+Prints one JSON line: Mcells/s over K timed steps (HIP events bracketing
+grid.run(K) on the current stream, after W warm-up steps), the HBM roofline of
+the dominant kernel (algorithmic bytes per launch / average launch time,
+timed live with HIP events on the launch stream), and a CPU baseline: the
+numpy oracle on the same configuration, a bounded number of steps, one core.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0
+METRIC = "Mcells/s + % HBM roofline (256³ PML, fp64 & fp32); adjoint/forward time ratio"
+
+
+def sphere_permittivity(n=256, count=24, seed=0):
+    """Relative permittivity with `count` seeded spheres (centres uniform in
+    the grid, integer radius in [8, 32], eps uniform in [1, 12]; later spheres
+    overwrite earlier ones)."""
+    rng = np.random.default_rng(seed)
+    eps = np.ones((n, n, n))
+    ax = np.arange(n)
+    for _ in range(count):
+        c = rng.uniform(0, n, size=3)
+        r = int(rng.integers(8, 33))
+        e = float(rng.uniform(1.0, 12.0))
+        lo = np.maximum(np.floor(c - r).astype(int), 0)
+        hi = np.minimum(np.ceil(c + r).astype(int) + 1, n)
+        x = ax[lo[0]:hi[0], None, None]
+        y = ax[None, lo[1]:hi[1], None]
+        z = ax[None, None, lo[2]:hi[2]]
+        mask = (x - c[0]) ** 2 + (y - c[1]) ** 2 + (z - c[2]) ** 2 <= r * r
+        eps[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]][mask] = e
+    return eps
+
+
+PML_KEYS = lambda t: [(slice(0, t), slice(None), slice(None)), (slice(-t, None), slice(None), slice(None)),
+                      (slice(None), slice(0, t), slice(None)), (slice(None), slice(-t, None), slice(None)),
+                      (slice(None), slice(None), slice(0, t)), (slice(None), slice(None), slice(-t, None))]
+
+
+def build_grid(fdtd, n=256, pml=16):
+    eps = sphere_permittivity(n)
+    grid = fdtd.Grid((n, n, n), grid_spacing=50e-9, permittivity=eps, courant_number=3 ** -0.5)
+    for k in PML_KEYS(pml):
+        grid[k] = fdtd.PML()
+    grid[n // 2, n // 2, n // 2] = fdtd.PointSource(period=1550e-9 / 299792458.0, pulse=True)
+    return grid
+
+
+def algorithmic_bytes_per_half_step(n, pml, itemsize):
+    """12 values per cell (read field, other field, material; write field) plus
+    4 per (face, slab cell) (read and write the two psi components)."""
+    cells = n ** 3
+    slab = pml * n * n
+    return cells * 12 * itemsize + 6 * slab * 4 * itemsize
+
+
+def cpu_baseline(n, pml, steps):
+    """The numpy oracle (single-threaded ufuncs) on the same workload."""
+    from oracle import fdtd_oracle as orc
+
+    sim = orc.Sim((n, n, n), grid_spacing=50e-9, permittivity=sphere_permittivity(n), courant_number=3 ** -0.5)
+    for k in PML_KEYS(pml):
+        sim.add("PML", k)
+    sim.add("PointSource", (n // 2, n // 2, n // 2), period=1550e-9 / 299792458.0, pulse=True)
+    sim.step()  # first step outside the timing (page faults of fresh arrays)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        sim.step()
+    dt = time.perf_counter() - t0
+    return dict(value=n ** 3 * steps / dt / 1e6, unit="Mcells/s", cores=1, kind="port",
+                sample=f"{steps} steps of the {n}^3 workload (after 1 untimed step), numpy oracle, "
+                       f"{os.cpu_count()} host cores visible, ufuncs single-threaded")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=50)
+    ap.add_argument("--dtype", default="float64", choices=["float64", "float32"])
+    ap.add_argument("--n", type=int, default=256)
+    ap.add_argument("--pml", type=int, default=16)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--kernel-reps", type=int, default=50)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        sys.stderr.write("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)\n")
+        sys.exit(2)
+
+    import torch
+
+    import fdtd_amd as fdtd
+
+    fdtd.set_backend(f"hip.{args.dtype}")
+    n, pml = args.n, args.pml
+    grid = build_grid(fdtd, n, pml)
+    grid.run(args.warmup, progress_bar=False)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    grid.run(args.steps, progress_bar=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    ms_step = ms / args.steps
+    mcells = n ** 3 * args.steps / (ms / 1e3) / 1e6
+    itemsize = 8 if args.dtype == "float64" else 4
+    half_bytes = algorithmic_bytes_per_half_step(n, pml, itemsize)
+    step_frac = 2 * half_bytes / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9)
+
+    # dominant kernel: time the fused main kernels alone (events on the launch stream)
+    plan = grid._ensure_plan()
+    per_launch = {}
+    for is_H, nm in ((0, "k_main_E"), (1, "k_main_H")):
+        plan.bench_main_kernel(is_H, 5, stream.cuda_stream)
+        per_launch[nm] = plan.bench_main_kernel(is_H, args.kernel_reps, stream.cuda_stream) / args.kernel_reps
+    dom = max(per_launch, key=per_launch.get)
+    achieved = half_bytes / (per_launch[dom] / 1e3) / 1e9
+
+    base = None
+    if not args.no_cpu_baseline:
+        base = cpu_baseline(n, pml, args.cpu_steps)
+
+    dt = "f64" if args.dtype == "float64" else "f32"
+    out = {
+        "metric": METRIC,
+        "value": round(mcells, 1),
+        "unit": "Mcells/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": dt,
+        "data": "synthetic",
+        "config": {"workload": f"{n}^3 grid, {pml}-cell PML x6, 24 dielectric spheres, pulsed point source, {dt}",
+                   "grid": [n, n, n], "pml": pml, "parallelism": "single GPU"},
+        "step_hbm_roofline_frac": round(step_frac, 4),
+        "roofline": {"bound": "hbm", "kernel": f"{dom}<{'double' if dt == 'f64' else 'float'}>",
+                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                     "algorithmic_bytes_per_launch": half_bytes,
+                     "avg_launch_ms": {k: round(v, 5) for k, v in per_launch.items()}},
+        "cpu_baseline": base,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

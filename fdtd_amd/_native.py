@@ -1,0 +1,167 @@
+"""ctypes binding of libfdtd_hip.so (C ABI: include/fdtd_hip.h).
+
+The library is loaded after torch so that it binds to the HIP runtime torch
+already mapped (same SONAME); a second HIP runtime in the process would make
+torch's stream handles meaningless to the library, so that case is refused.
+"""
+
+import ctypes
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
+
+F64, F32 = 0, 1
+OBJ_PLAIN, OBJ_ABSORBING, OBJ_ANISOTROPIC = 0, 1, 2
+SRC_POINT, SRC_LINE, SRC_PLANE = 0, 1, 2
+
+_lib = None
+
+_c_int, _c_i64, _c_dbl, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+_pi64 = ctypes.POINTER(ctypes.c_int64)
+_pdbl = ctypes.POINTER(ctypes.c_double)
+
+_SIGNATURES = {
+    "fdtd_version": ([], ctypes.c_char_p),
+    "fdtd_last_error": ([], ctypes.c_char_p),
+    "fdtd_plan_create": ([_c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)], _c_int),
+    "fdtd_plan_destroy": ([_vp], _c_int),
+    "fdtd_plan_add_pml": ([_vp, _c_int, _c_int, _c_int, _pdbl, _pdbl, _pdbl, _pdbl, _vp, _vp], _c_int),
+    "fdtd_plan_add_periodic": ([_vp, _c_int], _c_int),
+    "fdtd_plan_add_object": ([_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp], _c_int),
+    "fdtd_plan_add_source": ([_vp, _c_int, _c_int, _pi64, _pdbl, _c_dbl, _c_int, _c_int], _c_int),
+    "fdtd_plan_set_waveforms": ([_vp, _vp, _c_i64], _c_int),
+    "fdtd_plan_add_detector": ([_vp, _c_int, _pi64, _vp, _vp, _c_i64, _c_i64, _c_i64], _c_int),
+    "fdtd_plan_finalize": ([_vp], _c_int),
+    "fdtd_plan_detector_counts": ([_vp, _c_int, _pi64, _pi64], _c_int),
+    "fdtd_half_step_E": ([_vp, _c_i64, _vp], _c_int),
+    "fdtd_half_step_H": ([_vp, _c_i64, _vp], _c_int),
+    "fdtd_run": ([_vp, _c_i64, _c_i64, _vp], _c_int),
+    "fdtd_bench_main_kernel": ([_vp, _c_int, _c_i64, _vp, ctypes.POINTER(ctypes.c_float)], _c_int),
+}
+
+
+def exported_symbols():
+    return sorted(_SIGNATURES)
+
+
+def _hip_runtimes_mapped():
+    paths = set()
+    try:
+        with open("/proc/self/maps") as fh:
+            for line in fh:
+                if "libamdhip64" in line:
+                    paths.add(os.path.realpath(line.split()[-1]))
+    except OSError:
+        pass
+    return paths
+
+
+def lib():
+    """The loaded library; raises RuntimeError when it is not built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    import torch  # noqa: F401  (maps torch's HIP runtime first)
+
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"{LIB_PATH} is missing: build it with `python -m fdtd_amd.csrc.build` "
+            "(or __graft_entry__.build()); the HIP backend has no other implementation")
+    L = ctypes.CDLL(LIB_PATH)
+    for name, (args, res) in _SIGNATURES.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = res
+    rts = _hip_runtimes_mapped()
+    if len(rts) > 1:
+        raise RuntimeError(f"two HIP runtimes are mapped in this process: {sorted(rts)}")
+    _lib = L
+    return _lib
+
+
+def check(rc):
+    if rc != 0:
+        raise RuntimeError(f"libfdtd_hip: {lib().fdtd_last_error().decode()} (code {rc})")
+
+
+def _i64_array(values):
+    arr = (ctypes.c_int64 * len(values))(*[int(v) for v in values])
+    return arr
+
+
+def _dbl_array(values):
+    arr = (ctypes.c_double * len(values))(*[float(v) for v in values])
+    return arr
+
+
+class Plan:
+    """Owner of one FdtdPlan*; every device buffer it points at is a torch
+    tensor that the caller keeps alive for the plan's lifetime."""
+
+    def __init__(self, dtype_code, shape, courant, E, H, inv_eps, inv_mu):
+        L = lib()
+        self._L = L
+        h = _vp()
+        check(L.fdtd_plan_create(dtype_code, shape[0], shape[1], shape[2], float(courant),
+                                 E.data_ptr(), H.data_ptr(), inv_eps.data_ptr(), inv_mu.data_ptr(),
+                                 ctypes.byref(h)))
+        self._h = h
+        self._keep = [E, H, inv_eps, inv_mu]
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.fdtd_plan_destroy(h)
+            self._h = None
+
+    def add_pml(self, axis, high, thickness, bE, cE, bH, cH, psiE, psiH):
+        check(self._L.fdtd_plan_add_pml(self._h, axis, int(high), thickness, _dbl_array(bE), _dbl_array(cE),
+                                        _dbl_array(bH), _dbl_array(cH), psiE.data_ptr(), psiH.data_ptr()))
+        self._keep += [psiE, psiH]
+
+    def add_periodic(self, axis):
+        check(self._L.fdtd_plan_add_periodic(self._h, axis))
+
+    def add_object(self, kind, box, inv, absorption=None):
+        (x0, x1), (y0, y1), (z0, z1) = box
+        check(self._L.fdtd_plan_add_object(self._h, kind, x0, x1, y0, y1, z0, z1, inv.data_ptr(),
+                                           absorption.data_ptr() if absorption is not None else None))
+        self._keep += [inv] + ([absorption] if absorption is not None else [])
+
+    def add_source(self, kind, cells, profile, amplitude, comp_E, comp_H):
+        prof = _dbl_array(profile) if profile is not None else None
+        check(self._L.fdtd_plan_add_source(self._h, kind, len(cells), _i64_array(cells), prof, float(amplitude),
+                                           comp_E, comp_H))
+
+    def set_waveforms(self, table):
+        check(self._L.fdtd_plan_set_waveforms(self._h, table.data_ptr() if table is not None else None,
+                                              0 if table is None else table.shape[1]))
+        self._wave = table
+
+    def add_detector(self, cells, bufE, bufH, count_E, count_H):
+        check(self._L.fdtd_plan_add_detector(self._h, len(cells), _i64_array(cells), bufE.data_ptr(),
+                                             bufH.data_ptr(), bufE.shape[0], count_E, count_H))
+        self._keep += [bufE, bufH]
+
+    def finalize(self):
+        check(self._L.fdtd_plan_finalize(self._h))
+
+    def detector_counts(self, index):
+        e, h = ctypes.c_int64(), ctypes.c_int64()
+        check(self._L.fdtd_plan_detector_counts(self._h, index, ctypes.byref(e), ctypes.byref(h)))
+        return e.value, h.value
+
+    def half_step_E(self, q, stream):
+        check(self._L.fdtd_half_step_E(self._h, q, stream))
+
+    def half_step_H(self, q, stream):
+        check(self._L.fdtd_half_step_H(self._h, q, stream))
+
+    def run(self, q0, nsteps, stream):
+        check(self._L.fdtd_run(self._h, q0, nsteps, stream))
+
+    def bench_main_kernel(self, is_H, reps, stream):
+        ms = ctypes.c_float()
+        check(self._L.fdtd_bench_main_kernel(self._h, int(is_H), reps, stream, ctypes.byref(ms)))
+        return ms.value

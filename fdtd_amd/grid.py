@@ -1,0 +1,434 @@
+"""The FDTD grid on the HIP backend (API of fdtd/grid.py:80-514).
+
+State lives in HBM as component-major torch tensors, E and H of shape
+(3, Nx, Ny, Nz) (z fastest), and ``grid.E`` / ``grid.H`` /
+``grid.inverse_permittivity`` / ``grid.inverse_permeability`` are
+(Nx, Ny, Nz, 3) views of them, so user indexing such as
+``grid.E[x, y, z, 2] = v`` writes device memory directly and assignment
+``grid.E = array`` copies into it.
+
+``update_E`` / ``update_H`` / ``step`` / ``run`` do not walk the component
+hooks the way the reference does (grid.py:275-325); they hand the whole half
+step to libfdtd_hip, which applies the same updates in the same order inside
+fused kernels. The native plan is (re)built lazily whenever the registered
+components or the arrays they own change.
+"""
+
+import os
+from datetime import datetime
+from os import makedirs, path
+
+import numpy as np
+import torch
+
+from . import _native
+from . import constants as const
+from .backend import backend as bd
+from .typing_ import Number, Tuple
+
+
+def _soa_view(t):
+    return t.permute(1, 2, 3, 0)
+
+
+def curl_E(E):
+    """Forward-difference curl of an E-type (Nx,Ny,Nz,3) array (fdtd/grid.py:29-51).
+    Host-side helper for inspection; the time step never calls it."""
+    curl = torch.zeros_like(E) if torch.is_tensor(E) else np.zeros_like(E)
+    curl[:, :-1, :, 0] += E[:, 1:, :, 2] - E[:, :-1, :, 2]
+    curl[:, :, :-1, 0] -= E[:, :, 1:, 1] - E[:, :, :-1, 1]
+    curl[:, :, :-1, 1] += E[:, :, 1:, 0] - E[:, :, :-1, 0]
+    curl[:-1, :, :, 1] -= E[1:, :, :, 2] - E[:-1, :, :, 2]
+    curl[:-1, :, :, 2] += E[1:, :, :, 1] - E[:-1, :, :, 1]
+    curl[:, :-1, :, 2] -= E[:, 1:, :, 0] - E[:, :-1, :, 0]
+    return curl
+
+
+def curl_H(H):
+    """Backward-difference curl of an H-type (Nx,Ny,Nz,3) array (fdtd/grid.py:54-76).
+    Host-side helper for inspection; the time step never calls it."""
+    curl = torch.zeros_like(H) if torch.is_tensor(H) else np.zeros_like(H)
+    curl[:, 1:, :, 0] += H[:, 1:, :, 2] - H[:, :-1, :, 2]
+    curl[:, :, 1:, 0] -= H[:, :, 1:, 1] - H[:, :, :-1, 1]
+    curl[:, :, 1:, 1] += H[:, :, 1:, 0] - H[:, :, :-1, 0]
+    curl[1:, :, :, 1] -= H[1:, :, :, 2] - H[:-1, :, :, 2]
+    curl[1:, :, :, 2] += H[1:, :, :, 1] - H[:-1, :, :, 1]
+    curl[:, 1:, :, 2] -= H[:, 1:, :, 0] - H[:, :-1, :, 0]
+    return curl
+
+
+class Grid:
+    """The FDTD grid: fields, materials, registered components and the loop."""
+
+    def __init__(
+        self,
+        shape: Tuple[Number, Number, Number],
+        grid_spacing: float = 155e-9,
+        permittivity: float = 1.0,
+        permeability: float = 1.0,
+        courant_number: float = None,
+    ):
+        self.grid_spacing = float(grid_spacing)
+        self.Nx, self.Ny, self.Nz = self._handle_tuple(shape)
+        self.D = int(self.Nx > 1) + int(self.Ny > 1) + int(self.Nz > 1)
+        max_courant_number = float(self.D) ** (-0.5)
+        if courant_number is None:
+            self.courant_number = 0.99 * max_courant_number
+        elif courant_number > max_courant_number:
+            raise ValueError(f"courant_number {courant_number} too high for a {self.D}D simulation")
+        else:
+            self.courant_number = float(courant_number)
+        self.time_step = self.courant_number * self.grid_spacing / const.c
+
+        self.dtype = bd.float
+        dev = bd.device
+        soa = (3, self.Nx, self.Ny, self.Nz)
+        self._E = torch.zeros(soa, dtype=self.dtype, device=dev)
+        self._H = torch.zeros(soa, dtype=self.dtype, device=dev)
+        self._inv_eps = torch.empty(soa, dtype=self.dtype, device=dev)
+        self._inv_mu = torch.empty(soa, dtype=self.dtype, device=dev)
+        self._fill_inverse(self._inv_eps, permittivity)
+        self._fill_inverse(self._inv_mu, permeability)
+
+        self.time_steps_passed = 0
+        self.sources = []
+        self.boundaries = []
+        self.detectors = []
+        self.objects = []
+        self.folder = None
+
+        self._plan = None
+        self._plan_key = None
+        self._wave = None
+        self._wave_key = None
+
+    # ------------------------------------------------------------ materials
+    def _fill_inverse(self, dst, value):
+        """dst (3,Nx,Ny,Nz) <- 1 / value broadcast to (Nx,Ny,Nz,3), computed on
+        the host in the field dtype (grid.py:141-151)."""
+        npdt = bd.numpy_float
+        if torch.is_tensor(value):
+            value = value.detach().cpu().numpy()
+        v = np.asarray(value)
+        if v.ndim == 3:
+            v = v[..., None]
+        v = v.astype(npdt, copy=False)
+        if v.ndim == 0:
+            dst.fill_(float(np.ones((), npdt) / v))
+            return
+        if v.ndim == 4 and v.shape[3] == 1 and v.shape[:3] == (self.Nx, self.Ny, self.Nz):
+            inv = np.ones(v.shape[:3], npdt) / v[..., 0]
+            t = torch.from_numpy(np.ascontiguousarray(inv)).to(dst.device)
+            for c in range(3):
+                dst[c].copy_(t)
+            return
+        inv = np.ones((self.Nx, self.Ny, self.Nz, 3), npdt) / v
+        dst.copy_(torch.from_numpy(np.ascontiguousarray(np.moveaxis(inv, 3, 0))).to(dst.device))
+
+    def _set_soa(self, dst, value):
+        view = _soa_view(dst)
+        if torch.is_tensor(value) and value.data_ptr() == view.data_ptr() and value.stride() == view.stride():
+            return
+        if not torch.is_tensor(value):
+            value = torch.as_tensor(np.asarray(value))
+        view.copy_(value.to(device=dst.device, dtype=dst.dtype).expand_as(view))
+
+    @property
+    def E(self):
+        return _soa_view(self._E)
+
+    @E.setter
+    def E(self, value):
+        self._set_soa(self._E, value)
+
+    @property
+    def H(self):
+        return _soa_view(self._H)
+
+    @H.setter
+    def H(self, value):
+        self._set_soa(self._H, value)
+
+    @property
+    def inverse_permittivity(self):
+        return _soa_view(self._inv_eps)
+
+    @inverse_permittivity.setter
+    def inverse_permittivity(self, value):
+        self._set_soa(self._inv_eps, value)
+
+    @property
+    def inverse_permeability(self):
+        return _soa_view(self._inv_mu)
+
+    @inverse_permeability.setter
+    def inverse_permeability(self, value):
+        self._set_soa(self._inv_mu, value)
+
+    # ------------------------------------------------------- unit handling
+    def _handle_distance(self, distance: Number) -> int:
+        """transform a distance to an integer number of gridpoints (grid.py:171-175)"""
+        if not isinstance(distance, int):
+            return int(float(distance) / self.grid_spacing + 0.5)
+        return distance
+
+    def _handle_time(self, time: Number) -> int:
+        """transform a time value to an integer number of timesteps (grid.py:177-181)"""
+        if not isinstance(time, int):
+            return int(float(time) / self.time_step + 0.5)
+        return time
+
+    def _handle_tuple(self, shape):
+        if len(shape) != 3:
+            raise ValueError(
+                f"invalid grid shape {shape}\ngrid shape should be a 3D tuple containing floats or ints")
+        x, y, z = shape
+        return self._handle_distance(x), self._handle_distance(y), self._handle_distance(z)
+
+    def _handle_slice(self, s: slice) -> slice:
+        f = lambda v: v if not isinstance(v, float) else self._handle_distance(v)
+        return slice(f(s.start), f(s.stop), f(s.step))
+
+    def _handle_single_key(self, key):
+        try:
+            len(key)
+            return [self._handle_distance(k) for k in key]
+        except TypeError:
+            if isinstance(key, slice):
+                return self._handle_slice(key)
+            return [self._handle_distance(key)]
+
+    @property
+    def x(self) -> float:
+        return self.Nx * self.grid_spacing
+
+    @property
+    def y(self) -> float:
+        return self.Ny * self.grid_spacing
+
+    @property
+    def z(self) -> float:
+        return self.Nz * self.grid_spacing
+
+    @property
+    def shape(self) -> Tuple[int, int, int]:
+        return (self.Nx, self.Ny, self.Nz)
+
+    @property
+    def time_passed(self) -> float:
+        return self.time_steps_passed * self.time_step
+
+    # ------------------------------------------------------------- plan
+    def _cell(self, x, y, z):
+        """Linear cell index with numpy's negative-index rule; IndexError if outside."""
+        out = []
+        for v, n in ((x, self.Nx), (y, self.Ny), (z, self.Nz)):
+            v = int(v)
+            if v < -n or v >= n:
+                raise IndexError(f"index {v} is out of bounds for a grid axis of size {n}")
+            out.append(v + n if v < 0 else v)
+        return (out[0] * self.Ny + out[1]) * self.Nz + out[2]
+
+    def _components(self):
+        return list(self.boundaries) + list(self.objects) + list(self.sources) + list(self.detectors)
+
+    def _current_plan_key(self):
+        key = [len(self.boundaries), len(self.objects), len(self.sources), len(self.detectors), self.courant_number]
+        for comp in self._components():
+            key.append(id(comp))
+            key.extend(comp._plan_key())
+        return tuple(key)
+
+    def _ensure_plan(self):
+        if bd.device.type != "cuda":
+            raise RuntimeError("the HIP backend needs a HIP device to step (arrays are in host memory)")
+        key = self._current_plan_key()
+        if self._plan is not None and key == self._plan_key:
+            return self._plan
+        dtype_code = _native.F64 if self.dtype == torch.float64 else _native.F32
+        plan = _native.Plan(dtype_code, (self.Nx, self.Ny, self.Nz), self.courant_number,
+                            self._E, self._H, self._inv_eps, self._inv_mu)
+        for comp in self._components():
+            comp._add_to_plan(plan)
+        self._plan = plan
+        self._plan_key = self._current_plan_key()
+        self._wave_key = None
+        self._ensure_waveforms(self.time_steps_passed + 1)
+        plan.finalize()
+        return plan
+
+    def _ensure_waveforms(self, q_end):
+        """Device table of per-step source values for q < q_end, computed on the
+        host with Python math (the values the reference computes each step)."""
+        if not self.sources:
+            self._plan.set_waveforms(None)
+            return
+        cap = 0 if self._wave is None else self._wave.shape[1]
+        key = tuple(id(s) for s in self.sources)
+        if self._wave is not None and key == self._wave_key and cap >= q_end:
+            return
+        new_cap = max(int(q_end), 2 * cap if key == self._wave_key else 0, 256)
+        table = np.empty((len(self.sources), new_cap), dtype=np.float64)
+        for r, src in enumerate(self.sources):
+            table[r] = [src._waveform(q) for q in range(new_cap)]
+        self._wave = torch.from_numpy(table).to(self._E.device)
+        self._wave_key = key
+        self._plan.set_waveforms(self._wave)
+
+    def _prepare(self, nsteps, e_steps, h_steps):
+        plan = self._ensure_plan()
+        grown = False
+        for det in self.detectors:
+            grown |= det._reserve(max(e_steps, h_steps))
+        if grown:
+            plan = self._ensure_plan()
+        self._ensure_waveforms(self.time_steps_passed + nsteps)
+        return plan
+
+    @staticmethod
+    def _stream():
+        return torch.cuda.current_stream().cuda_stream
+
+    # ------------------------------------------------------------- the loop
+    def run(self, total_time: Number, progress_bar: bool = True):
+        """run an FDTD simulation (grid.py:250-265).
+
+        The whole run is queued on the current HIP stream without host
+        synchronisation; with progress_bar the queueing is split into chunks
+        so the bar advances as work is submitted."""
+        if isinstance(total_time, float):
+            total_time /= self.time_step
+        n = int(total_time)
+        if n <= 0:
+            return
+        if type(self).step is not Grid.step:
+            # a subclass redefined step(): honour it step by step
+            it = range(n)
+            if progress_bar:
+                from tqdm import tqdm
+                it = tqdm(it)
+            for _ in it:
+                self.step()
+            return
+        chunks = [n]
+        bar = None
+        if progress_bar:
+            from tqdm import tqdm
+            bar = tqdm(total=n)
+            per = max(1, n // 100)
+            chunks = [per] * (n // per) + ([n % per] if n % per else [])
+        for m in chunks:
+            plan = self._prepare(m, m, m)
+            plan.run(self.time_steps_passed, m, self._stream())
+            for det in self.detectors:
+                det._count_E += m
+                det._count_H += m
+            self.time_steps_passed += m
+            if bar is not None:
+                bar.update(m)
+        if bar is not None:
+            bar.close()
+
+    def step(self):
+        """E half step, H half step, then advance the time index (grid.py:267-273)."""
+        self.update_E()
+        self.update_H()
+        self.time_steps_passed += 1
+
+    def update_E(self):
+        """Everything the reference's update_E does, in one fused launch
+        sequence (grid.py:275-299)."""
+        plan = self._prepare(1, 1, 0)
+        plan.half_step_E(self.time_steps_passed, self._stream())
+        for det in self.detectors:
+            det._count_E += 1
+
+    def update_H(self):
+        """Everything the reference's update_H does (grid.py:301-325)."""
+        plan = self._prepare(1, 0, 1)
+        plan.half_step_H(self.time_steps_passed, self._stream())
+        for det in self.detectors:
+            det._count_H += 1
+
+    def reset(self):
+        """zero E, H and the time index; PML state and detector traces are kept
+        (grid.py:327-331)"""
+        self.H *= 0.0
+        self.E *= 0.0
+        self.time_steps_passed *= 0
+
+    def add_source(self, name, source):
+        """the reference's broken helper (grid.py:333-336) is not carried over"""
+        raise NotImplementedError("register components with grid[x, y, z] = component")
+
+    add_boundary = add_detector = add_object = add_source
+
+    def promote_dtypes_to_complex(self):
+        raise NotImplementedError("complex fields are not supported by the HIP backend")
+
+    def __setitem__(self, key, attr):
+        if not isinstance(key, tuple):
+            x, y, z = key, slice(None), slice(None)
+        elif len(key) == 1:
+            x, y, z = key[0], slice(None), slice(None)
+        elif len(key) == 2:
+            x, y, z = key[0], key[1], slice(None)
+        elif len(key) == 3:
+            x, y, z = key
+        else:
+            raise KeyError("maximum number of indices for the grid is 3")
+        attr._register_grid(
+            grid=self,
+            x=self._handle_single_key(x),
+            y=self._handle_single_key(y),
+            z=self._handle_single_key(z),
+        )
+
+    def __repr__(self):
+        return (
+            f"{self.__class__.__name__}(shape=({self.Nx},{self.Ny},{self.Nz}), "
+            f"grid_spacing={self.grid_spacing:.2e}, courant_number={self.courant_number:.2f})"
+        )
+
+    def __str__(self):
+        s = repr(self) + "\n"
+        for title, comps in (("sources", self.sources), ("detectors", self.detectors),
+                             ("boundaries", self.boundaries), ("objects", self.objects)):
+            if comps:
+                s = s + f"\n{title}:\n"
+                for c in comps:
+                    s += str(c)
+        return s
+
+    # --------------------------------------------------------------- output
+    def save_simulation(self, sim_name=None):
+        """Create fdtd_output/fdtd_output_<timestamp> (grid.py:407-439)."""
+        makedirs("fdtd_output", exist_ok=True)
+        now = datetime.now()
+        full_sim_name = "-".join(str(v) for v in (now.year, now.month, now.day, now.hour, now.minute, now.second))
+        if sim_name is not None:
+            full_sim_name = full_sim_name + " (" + sim_name + ")"
+        self.folder = os.path.abspath(path.join("fdtd_output", "fdtd_output_" + full_sim_name))
+        self.full_sim_name = full_sim_name
+        makedirs(self.folder, exist_ok=True)
+        return self.folder
+
+    def save_data(self):
+        """Detector traces to <folder>/detector_readings.npz with keys
+        "<name> (E)" / "<name> (H)" (grid.py:490-514); one device-to-host
+        copy per trace."""
+        if self.folder is None:
+            raise Exception(
+                "Save location not initialized. Please read about 'fdtd.Grid.saveSimulation()' or try running "
+                "'grid.saveSimulation()'.")
+        dic = {}
+        for detector in self.detectors:
+            dic[str(detector.name) + " (E)"] = detector.trace("E").cpu().numpy()
+            dic[str(detector.name) + " (H)"] = detector.trace("H").cpu().numpy()
+        np.savez(path.join(self.folder, "detector_readings"), **dic)
+
+    def generate_video(self, delete_frames=False):
+        raise NotImplementedError("video generation (matplotlib + ffmpeg) is not part of the HIP backend")
+
+    def visualize(self, *args, **kwargs):
+        raise NotImplementedError("visualization is not part of the HIP backend; use bd.numpy(grid.E)")

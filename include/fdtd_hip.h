@@ -1,0 +1,129 @@
+/*
+ * fdtd_hip.h -- C ABI of libfdtd_hip.so, the MI355X (gfx950) FDTD step engine.
+ *
+ * The engine replaces the per-statement array work of the reference's
+ * Grid.update_E / Grid.update_H (fdtd/grid.py:275-325) and of the component
+ * hooks those call, with fused HIP kernels. A plan is built the way the
+ * reference registers components (grid[...] = component, fdtd/grid.py:358-375):
+ * create it for the grid arrays, then add boundaries, objects, sources and
+ * detectors in registration order, then finalize. Stepping is asynchronous on
+ * the caller's HIP stream; nothing synchronises the host per step.
+ *
+ * Conventions
+ *   - All array arguments are device pointers borrowed from the caller (the
+ *     Python layer keeps them alive as torch tensors); index lists and
+ *     coefficient profiles marked "host" are copied at call time.
+ *   - Fields E, H and the grid materials are stored component-major:
+ *     (3, Nx, Ny, Nz), z fastest. Object arrays keep the reference layout
+ *     (nx, ny, nz, 3) or (nx, ny, nz, 3, 3).
+ *   - dtype: FDTD_F64 or FDTD_F32 for every floating array of the plan.
+ *   - Every function returns 0 on success and a negative code on error;
+ *     fdtd_last_error() then describes the failure. No C++ exception crosses
+ *     the ABI.
+ *   - A plan is used from one host thread at a time.
+ */
+#ifndef FDTD_HIP_H
+#define FDTD_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct FdtdPlan FdtdPlan;
+
+enum { FDTD_F64 = 0, FDTD_F32 = 1 };
+enum { FDTD_OBJ_PLAIN = 0, FDTD_OBJ_ABSORBING = 1, FDTD_OBJ_ANISOTROPIC = 2 };
+enum { FDTD_SRC_POINT = 0, FDTD_SRC_LINE = 1, FDTD_SRC_PLANE = 2 };
+
+/* Version string of the library build. */
+const char* fdtd_version(void);
+
+/* Text of the last error raised on the calling thread ("" if none). */
+const char* fdtd_last_error(void);
+
+/* New plan over a grid of nx*ny*nz cells.
+ * Replaces the state set up by Grid.__init__ (fdtd/grid.py:90-169):
+ * E, H (3,nx,ny,nz) read-write; inv_eps, inv_mu (3,nx,ny,nz) read-only,
+ * read every step so that writes to the grid materials take effect. */
+int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number,
+                     void* E, void* H, const void* inv_eps, const void* inv_mu,
+                     FdtdPlan** out);
+
+int fdtd_plan_destroy(FdtdPlan* plan);
+
+/* One CPML face (fdtd/boundaries.py:225-487 with the face subclasses
+ * :490-625). axis 0/1/2, high 0 (low face) or 1 (high face), thickness t.
+ * bE, cE, bH, cH: host float64 profiles of length t along the normal axis
+ * (the only non-zero CPML component). psiE, psiH: device state of the face,
+ * 2 components x slab cells, slab laid out like the grid with the normal
+ * axis shortened to t, in the plan dtype. The two components are the two
+ * field components other than `axis`, in increasing order. */
+int fdtd_plan_add_pml(FdtdPlan* plan, int axis, int high, int thickness,
+                      const double* bE, const double* cE,
+                      const double* bH, const double* cH,
+                      void* psiE, void* psiH);
+
+/* Periodic boundary along `axis`: E[0] = E[-1] after the E update and
+ * H[-1] = H[0] after the H update (fdtd/boundaries.py:143-219). */
+int fdtd_plan_add_periodic(FdtdPlan* plan, int axis);
+
+/* Object over the box [x0,x1) x [y0,y1) x [z0,z1) (fdtd/objects.py:24-277).
+ * inv: the object's inverse permittivity, (nx,ny,nz,3) for PLAIN and
+ * ABSORBING, (nx,ny,nz,3,3) for ANISOTROPIC. absorption: (nx,ny,nz,3)
+ * absorption factors for ABSORBING (objects.py:198-205), else NULL. */
+int fdtd_plan_add_object(FdtdPlan* plan, int kind, int x0, int x1, int y0, int y1,
+                         int z0, int z1, const void* inv, const void* absorption);
+
+/* Source over n cells (fdtd/sources.py:25-501). cells: host linear cell
+ * indices ((i*ny + j)*nz + k). POINT adds the waveform value to E[comp_E];
+ * LINE adds profile[p] * waveform to E[comp_E]; PLANE assigns
+ * amplitude * waveform to E[comp_E] and, in the H half step, to H[comp_H].
+ * profile: host float64[n] for LINE, else NULL. The waveform of source s at
+ * step q is read from the table set by fdtd_plan_set_waveforms. */
+int fdtd_plan_add_source(FdtdPlan* plan, int kind, int n, const int64_t* cells,
+                         const double* profile, double amplitude, int comp_E, int comp_H);
+
+/* Device table of float64 waveform values, row s (source registration order)
+ * holding q = 0 .. capacity-1 (fdtd/sources.py:93-109, 278-297, 476-486). */
+int fdtd_plan_set_waveforms(FdtdPlan* plan, const double* table, int64_t capacity);
+
+/* Detector sampling the 3 field components at n cells after every E half step
+ * and every H half step (fdtd/detectors.py:20-280). bufE, bufH: device
+ * (capacity, n, 3) arrays in the plan dtype; count_E / count_H samples are
+ * already stored in them. */
+int fdtd_plan_add_detector(FdtdPlan* plan, int n, const int64_t* cells,
+                           void* bufE, void* bufH, int64_t capacity,
+                           int64_t count_E, int64_t count_H);
+
+/* Upload the plan tables; required before stepping. */
+int fdtd_plan_finalize(FdtdPlan* plan);
+
+/* Samples stored so far by detector `index`. */
+int fdtd_plan_detector_counts(const FdtdPlan* plan, int index, int64_t* count_E, int64_t* count_H);
+
+/* One E half step at time index q: everything Grid.update_E does
+ * (fdtd/grid.py:275-299). stream: hipStream_t (0 = default stream). */
+int fdtd_half_step_E(FdtdPlan* plan, int64_t q, void* stream);
+
+/* One H half step at time index q (fdtd/grid.py:301-325). */
+int fdtd_half_step_H(FdtdPlan* plan, int64_t q, void* stream);
+
+/* nsteps full steps starting at time index q0 (fdtd/grid.py:250-273):
+ * E half step then H half step at q, then q+1. */
+int fdtd_run(FdtdPlan* plan, int64_t q0, int64_t nsteps, void* stream);
+
+/* Measurement hook, no counterpart in the reference (used by bench.py to
+ * price the dominant kernel against the HBM roofline): launch the fused main
+ * kernel of one half step (is_H 0: E, 1: H) `reps` times back to back between
+ * two HIP events recorded on `stream`, wait for the end event and store the
+ * elapsed milliseconds in *ms. The launches advance the fields (without
+ * periodic stages, sources or detectors), so use it on a scratch state. */
+int fdtd_bench_main_kernel(FdtdPlan* plan, int is_H, int64_t reps, void* stream, float* ms);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FDTD_HIP_H */

@@ -332,3 +332,100 @@ def build_oracle(oracle_mod, sc):
         sim.E[...] = init["E"]
         sim.H[...] = init["H"]
     return sim, comps
+
+
+def random_scenario(seed, steps=15):
+    """A random but valid configuration: random grid (sometimes 2D), random
+    subset and registration order of PML faces, an optional periodic boundary
+    at a random position among them, random objects of every kind, point /
+    line / plane sources, line / block detectors, random initial fields."""
+    rs = np.random.RandomState(1000 + seed)
+    two_d = rs.rand() < 0.25
+    shape = [int(rs.randint(6, 17)), int(rs.randint(6, 15)), 1 if two_d else int(rs.randint(6, 15))]
+    D = sum(n > 1 for n in shape)
+    s = slice
+    full = [s(None)] * 3
+    bnds = []
+    per_axis = int(rs.randint(0, 3)) if rs.rand() < 0.5 else None
+    for axis in range(3):
+        if shape[axis] < 6 or axis == per_axis:
+            continue
+        for high in (False, True):
+            if rs.rand() < 0.7:
+                t = int(rs.randint(2, max(3, shape[axis] // 3 + 1)))
+                k = list(full)
+                k[axis] = s(-t, None) if high else s(0, t)
+                bnds.append(("PML", dict(a=float(rs.choice([1e-8, 3e-8]))), tuple(k), {}))
+    rs.shuffle(bnds)
+    if per_axis is not None:
+        k = list(full)
+        k[per_axis] = int(rs.choice([0, -1]))
+        bnds.insert(int(rs.randint(0, len(bnds) + 1)), ("PeriodicBoundary", dict(), tuple(k), {}))
+
+    def box():
+        lo = [int(rs.randint(0, n)) for n in shape]
+        hi = [min(n, l + int(rs.randint(1, 6))) for l, n in zip(lo, shape)]
+        return lo, hi, tuple(s(a, b) for a, b in zip(lo, hi))
+
+    objs = []
+    for _ in range(int(rs.randint(0, 4))):
+        lo, hi, key = box()
+        kind = ["Object", "AbsorbingObject", "AnisotropicObject"][int(rs.randint(0, 3))]
+        dims = tuple(b - a for a, b in zip(lo, hi))
+        perm = float(rs.uniform(1.2, 4.0)) if rs.rand() < 0.5 else rs.uniform(1.2, 4.0, size=dims)
+        p = dict(permittivity=perm)
+        over = {}
+        if kind == "AbsorbingObject":
+            p["conductivity"] = float(rs.choice([1e3, 5e4, 1e6]))
+        if kind == "AnisotropicObject" and rs.rand() < 0.6:
+            over["inverse_permittivity"] = _spd_inverse_field(rs, dims, 1.0, 5.0)
+        objs.append((kind, p, key, over))
+
+    srcs = []
+    pt = tuple(int(rs.randint(0, n)) for n in shape)
+    srcs.append(("PointSource", dict(period=int(rs.randint(6, 20)), pulse=bool(rs.rand() < 0.5),
+                                     phase_shift=float(rs.uniform(0, 3))), pt, {}))
+    if rs.rand() < 0.8:
+        lo = [int(rs.randint(0, n)) for n in shape]
+        hi = [int(rs.randint(0, n)) for n in shape]
+        ax = int(np.argmax(shape))
+        lo[ax], hi[ax] = 0, shape[ax]
+        key = tuple(s(a, b) if a != b else a for a, b in zip(lo, hi))
+        srcs.append(("LineSource", dict(period=int(rs.randint(6, 20)), pulse=bool(rs.rand() < 0.5),
+                                        amplitude=float(rs.uniform(0.5, 2))), key, {}))
+    flat = [a for a in range(3) if shape[a] > 1]
+    if len(flat) == 3 and rs.rand() < 0.7:
+        n_ax = int(rs.choice(flat))
+        pol = [a for a in range(3) if a != n_ax][int(rs.randint(0, 2))]
+        k = [s(1, n - 1) for n in shape]
+        k[n_ax] = int(rs.randint(0, shape[n_ax]))
+        srcs.append(("PlaneSource", dict(period=int(rs.randint(6, 20)), amplitude=float(rs.uniform(0.1, 1)),
+                                         polarization="xyz"[pol]), tuple(k), {}))
+    dets = []
+    ax = int(np.argmax(shape))
+    k = [int(rs.randint(0, n)) for n in shape]
+    k[ax] = s(None)
+    dets.append(("LineDetector", dict(), tuple(k), {}))
+    lo = [int(rs.randint(0, n - 1)) if n > 1 else 0 for n in shape]
+    k = tuple(s(a, min(a + 1, n - 1)) for a, n in zip(lo, shape))
+    dets.append(("BlockDetector", dict(), k, {}))
+
+    comps = []
+    groups = [bnds, objs, srcs, dets]
+    while any(groups):
+        g = groups[int(rs.randint(0, 4))]
+        if g:
+            comps.append(g.pop(0))
+    eps = rs.uniform(1.0, 4.0, size=tuple(shape)) if rs.rand() < 0.7 else 2.0
+    mu = rs.uniform(1.0, 2.0, size=tuple(shape) + (3,)) if rs.rand() < 0.4 else 1.0
+    return dict(
+        name=f"random_{seed}",
+        shape=tuple(shape),
+        grid_spacing=float(rs.choice([50e-9, 100e-9])),
+        courant_number=0.95 * D ** -0.5,
+        permittivity=eps,
+        permeability=mu,
+        init=dict(E=0.1 * rs.randn(*shape, 3), H=0.1 * rs.randn(*shape, 3)),
+        components=comps,
+        program=[("run", steps)],
+    )

@@ -1,0 +1,222 @@
+"""Parity of the HIP path (hip.float64 / hip.float32) with the reference.
+
+  * every golden scenario: the device run against the fixture the reference
+    wrote (fields, detector traces, PML state), bit-exact in float64
+  * random scenarios: the device run against the numpy oracle, bit-exact
+  * the 256^3 configuration with 16-cell PML: a few steps against the oracle,
+    plus size-independent properties at full size (exact linearity)
+  * float32: error against the float64 fixture, stated and bounded
+
+Tolerances: the float64 bar is <= 1e-12 relative L2 on all fields; the
+kernels are written to reproduce numpy's rounding sequence, so the tests
+also demand bitwise equality.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import fdtd_amd as fdtd
+import scenarios
+from oracle import fdtd_oracle as orc
+
+pytestmark = pytest.mark.gpu
+
+SCENARIOS = [sc["name"] for sc in scenarios.all_scenarios()]
+PSI_NAMES = {0: ("Ey", "Ez", "Hy", "Hz"), 1: ("Ex", "Ez", "Hx", "Hz"), 2: ("Ex", "Ey", "Hx", "Hy")}
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def rel_l2(a, b):
+    d = np.linalg.norm((np.asarray(a, np.float64) - np.asarray(b, np.float64)).ravel())
+    n = np.linalg.norm(np.asarray(b, np.float64).ravel())
+    return d / n if n > 0 else d
+
+
+def _build(sc):
+    grid, comps = scenarios.build_fdtd(fdtd, sc, to_array=torch.from_numpy)
+    return grid, comps
+
+
+def _np(t):
+    return t.detach().cpu().numpy()
+
+
+def _assert_same(got, ref, what, bitwise=True):
+    r = rel_l2(got, ref)
+    assert r <= 1e-12, f"{what}: relative L2 {r:.3e}"
+    if bitwise:
+        assert np.array_equal(got, ref), f"{what}: not bit-identical (rel L2 {r:.3e}, max {np.abs(got - ref).max():.3e})"
+
+
+@pytest.fixture(autouse=True)
+def _fp64():
+    fdtd.set_backend("hip.float64")
+    yield
+    fdtd.set_backend("hip.float64")
+
+
+@pytest.mark.parametrize("name", SCENARIOS)
+def test_float64_matches_reference_fixture(golden, name):
+    _say(f"[gpu] fixture scenario {name}")
+    g = golden(name)
+    sc = scenarios.get_scenario(name)
+    grid, comps = _build(sc)
+    scenarios.run_program(grid, sc)
+    torch.cuda.synchronize()
+    assert grid.time_steps_passed == int(g["time_steps_passed"])
+    _assert_same(_np(grid.E), g["E"], "E")
+    _assert_same(_np(grid.H), g["H"], "H")
+    for i, c in enumerate(comps):
+        p = f"out/{i}/"
+        cls = str(g[f"reg/{i}/class"])
+        if cls.startswith("_PML"):
+            for q, nm in enumerate(PSI_NAMES[c.axis]):
+                store = c._psiE if nm[0] == "E" else c._psiH
+                _assert_same(_np(store[q % 2]), g[p + "psi_" + nm], f"psi_{nm} of face {i}")
+        elif cls in ("LineDetector", "BlockDetector"):
+            _assert_same(_np(c.trace("E")), g[p + "E"], f"detector {i} E")
+            _assert_same(_np(c.trace("H")), g[p + "H"], f"detector {i} H")
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_float64_random_scenarios_match_oracle(seed):
+    sc = scenarios.random_scenario(seed)
+    _say(f"[gpu] random scenario {seed}: shape {sc['shape']}, {[c[0] for c in sc['components']]}")
+    sim, ocomps = scenarios.build_oracle(orc, sc)
+    scenarios.run_program(sim, sc)
+    grid, comps = _build(sc)
+    scenarios.run_program(grid, sc)
+    torch.cuda.synchronize()
+    # numpy's einsum association is machine-dependent; anisotropic cases at tolerance
+    bitwise = not any(k[0] == "AnisotropicObject" for k in sc["components"])
+    _assert_same(_np(grid.E), sim.E, "E", bitwise)
+    _assert_same(_np(grid.H), sim.H, "H", bitwise)
+    dets = [c for c in comps if isinstance(c, (fdtd.LineDetector, fdtd.BlockDetector))]
+    odets = sim.detectors
+    for d, o in zip(dets, odets):
+        _assert_same(_np(d.trace("E")).reshape(np.array(o.E).shape), np.array(o.E), "detector E", bitwise)
+        _assert_same(_np(d.trace("H")).reshape(np.array(o.H).shape), np.array(o.H), "detector H", bitwise)
+
+
+def test_step_by_step_equals_run():
+    """grid.step() in a loop, grid.run() and manual half steps give identical bits."""
+    sc = scenarios.get_scenario("g2")
+    out = []
+    for mode in ("run", "step", "half"):
+        grid, comps = _build(sc)
+        for _ in range(2):
+            if mode == "run":
+                grid.run(7, progress_bar=False)
+            elif mode == "step":
+                for _ in range(7):
+                    grid.step()
+            else:
+                for _ in range(7):
+                    grid.update_E()
+                    grid.update_H()
+                    grid.time_steps_passed += 1
+        torch.cuda.synchronize()
+        out.append((_np(grid.E), _np(grid.H), _np(comps[-1].trace("E"))))
+    for other in out[1:]:
+        for a, b in zip(out[0], other):
+            assert np.array_equal(a, b)
+
+
+def test_material_edits_take_effect():
+    """Writes to grid materials and rebinding an object's array are honoured
+    on the next step, as the reference reads them every step."""
+    sc = scenarios.get_scenario("g2")
+    sim, ocomps = scenarios.build_oracle(orc, sc)
+    grid, comps = _build(sc)
+    for g_ in (grid, sim):
+        g_.run(5) if g_ is sim else g_.run(5, progress_bar=False)
+    sim.inverse_permittivity[3:9, 2:6, 4:10] *= 0.5
+    grid.inverse_permittivity[3:9, 2:6, 4:10] *= 0.5
+    new_inv = np.asarray(ocomps[6].inverse_permittivity) * 0.75
+    ocomps[6].inverse_permittivity = new_inv.copy()
+    comps[6].inverse_permittivity = torch.from_numpy(new_inv.copy())
+    sim.run(5)
+    grid.run(5, progress_bar=False)
+    torch.cuda.synchronize()
+    _assert_same(_np(grid.E), sim.E, "E", bitwise=False)
+    _assert_same(_np(grid.H), sim.H, "H", bitwise=False)
+
+
+def _c2_grid(mod_grid, n=256, pml=16, amplitude=1.0, oracle=False):
+    """The 256^3 benchmark configuration: 24 seeded dielectric spheres,
+    16-cell PML on all six faces, pulsed 1550 nm point source at the centre."""
+    from bench import sphere_permittivity
+    eps = sphere_permittivity(n)
+    kw = dict(grid_spacing=50e-9, permittivity=eps, courant_number=3 ** -0.5)
+    grid = mod_grid.Sim((n, n, n), **kw) if oracle else fdtd.Grid((n, n, n), **kw)
+    keys = [(slice(0, pml), slice(None), slice(None)), (slice(-pml, None), slice(None), slice(None)),
+            (slice(None), slice(0, pml), slice(None)), (slice(None), slice(-pml, None), slice(None)),
+            (slice(None), slice(None), slice(0, pml)), (slice(None), slice(None), slice(-pml, None))]
+    src = dict(period=1550e-9 / 299792458.0, pulse=True, amplitude=amplitude)
+    if oracle:
+        for k in keys:
+            grid.add("PML", k)
+        grid.add("PointSource", (n // 2, n // 2, n // 2), **src)
+    else:
+        for k in keys:
+            grid[k] = fdtd.PML()
+        grid[n // 2, n // 2, n // 2] = fdtd.PointSource(**src)
+    return grid
+
+
+def test_full_size_256_matches_oracle_first_steps():
+    """The benchmark configuration itself: 3 steps from random fields against
+    the oracle at full size (bit-exact)."""
+    _say("[gpu] 256^3 parity vs oracle (3 steps)")
+    rs = np.random.RandomState(5)
+    e0 = rs.randn(256, 256, 256, 3) * 0.01
+    h0 = rs.randn(256, 256, 256, 3) * 0.01
+    sim = _c2_grid(orc, oracle=True)
+    sim.E[...] = e0
+    sim.H[...] = h0
+    grid = _c2_grid(None)
+    grid.E = torch.from_numpy(e0)
+    grid.H = torch.from_numpy(h0)
+    del e0, h0
+    sim.run(3)
+    grid.run(3, progress_bar=False)
+    torch.cuda.synchronize()
+    _assert_same(_np(grid.E), sim.E, "E")
+    _assert_same(_np(grid.H), sim.H, "H")
+
+
+def test_full_size_256_linearity_is_exact():
+    """Doubling the source amplitude doubles every field bit-for-bit (every
+    operation of the step is linear, and scaling by 2 is exact)."""
+    _say("[gpu] 256^3 linearity, 40 steps")
+    a = _c2_grid(None, amplitude=1.0)
+    a.run(40, progress_bar=False)
+    Ea = a._E.clone()
+    del a
+    b = _c2_grid(None, amplitude=2.0)
+    b.run(40, progress_bar=False)
+    torch.cuda.synchronize()
+    assert torch.count_nonzero(Ea).item() > 0
+    assert torch.equal(b._E, 2 * Ea)
+
+
+@pytest.mark.parametrize("name", ["g2", "c1", "periodic_y_last"])
+def test_float32_error_against_float64_reference(golden, name):
+    """float32 fields: relative L2 error against the float64 reference stated
+    (printed) and bounded by 1e-4."""
+    fdtd.set_backend("hip.float32")
+    g = golden(name)
+    sc = scenarios.get_scenario(name)
+    grid, comps = scenarios.build_fdtd(fdtd, sc, to_array=lambda a: torch.from_numpy(np.asarray(a, np.float32)))
+    scenarios.run_program(grid, sc)
+    torch.cuda.synchronize()
+    eE = rel_l2(_np(grid.E), g["E"])
+    eH = rel_l2(_np(grid.H), g["H"])
+    _say(f"[gpu] float32 {name}: relative L2 error E {eE:.2e} H {eH:.2e}")
+    assert eE < 1e-4 and eH < 1e-4

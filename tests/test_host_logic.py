@@ -1,0 +1,202 @@
+"""Host-side registration logic of fdtd_amd (no GPU): arrays are kept in host
+memory and every resolved index, profile and coefficient is compared with
+what the reference computed for the same scenario (tests/golden/*.npz)."""
+
+import numpy as np
+import pytest
+import torch
+
+import fdtd_amd as fdtd
+import scenarios
+from fdtd_amd.backend import host_only
+from fdtd_amd.boundaries import DomainBorderPML, pml_profiles
+
+SCENARIOS = [sc["name"] for sc in scenarios.all_scenarios()]
+
+
+@pytest.fixture(autouse=True)
+def _host_arrays():
+    host_only(True)
+    fdtd.set_backend("hip.float64")
+    yield
+    host_only(False)
+
+
+def _np(t):
+    return t.detach().cpu().numpy() if torch.is_tensor(t) else np.asarray(t)
+
+
+@pytest.mark.parametrize("name", SCENARIOS)
+def test_registration_matches_reference(golden, name):
+    g = golden(name)
+    sc = scenarios.get_scenario(name)
+    grid, comps = scenarios.build_fdtd(fdtd, sc, to_array=torch.from_numpy)
+    assert [grid.Nx, grid.Ny, grid.Nz] == list(g["reg/Nxyz"])
+    assert grid.courant_number == float(g["reg/courant_number"])
+    assert grid.time_step == float(g["reg/time_step"])
+    if "reg/grid_inverse_permittivity" in g.files:
+        assert np.array_equal(_np(grid.inverse_permittivity), g["reg/grid_inverse_permittivity"])
+    for i, c in enumerate(comps):
+        p = f"reg/{i}/"
+        cls = str(g[p + "class"])
+        assert type(c).__name__ == cls
+        if cls.startswith("_PML"):
+            assert c.thickness == int(g[p + "thickness"])
+            for nm in ("bE", "cE", "bH", "cH"):
+                assert np.array_equal(getattr(c, "_" + nm), g[p + nm]), nm
+                full = _np(getattr(c, nm))
+                other = [k for k in range(3) if k != c.axis][0]
+                assert full.flat[other] == float(g[p + nm + "_other"]) or (
+                    full.flat[other] == 0 and float(g[p + nm + "_other"]) == 0)
+        elif cls in ("Object", "AbsorbingObject", "AnisotropicObject"):
+            assert [c.x.start, c.x.stop] == list(g[p + "x"])
+            assert [c.y.start, c.y.stop] == list(g[p + "y"])
+            assert [c.z.start, c.z.stop] == list(g[p + "z"])
+            if cls != "AnisotropicObject":
+                assert np.array_equal(_np(c.inverse_permittivity), g[p + "inverse_permittivity"])
+            if cls == "AbsorbingObject":
+                assert np.array_equal(_np(c.absorption_factor), g[p + "absorption_factor"])
+        elif cls == "PointSource":
+            assert [c.x, c.y, c.z] == list(g[p + "xyz"])
+            assert c.period == int(g[p + "period"])
+        elif cls == "LineSource":
+            assert (c.x, c.y, c.z) == (list(g[p + "x"]), list(g[p + "y"]), list(g[p + "z"]))
+            assert c.period == int(g[p + "period"])
+            assert np.array_equal(c.profile, g[p + "profile"])
+        elif cls == "PlaneSource":
+            assert [c.x.start, c.x.stop, c.y.start, c.y.stop, c.z.start, c.z.stop] == \
+                list(g[p + "x"]) + list(g[p + "y"]) + list(g[p + "z"])
+            assert [c._Epol, c._Hpol] == list(g[p + "pol"])
+            assert c.period == int(g[p + "period"])
+        elif cls in ("LineDetector", "BlockDetector"):
+            assert (list(c.x), list(c.y), list(c.z)) == (list(g[p + "x"]), list(g[p + "y"]), list(g[p + "z"]))
+
+
+@pytest.mark.parametrize("name", SCENARIOS)
+def test_waveform_tables_match_oracle(name):
+    """The per-step source scalars the device table holds equal what the oracle
+    adds each step (same Python math expressions)."""
+    from oracle import fdtd_oracle as orc
+
+    sc = scenarios.get_scenario(name)
+    grid, comps = scenarios.build_fdtd(fdtd, sc, to_array=torch.from_numpy)
+    sim, ocomps = scenarios.build_oracle(orc, sc)
+    for c, o in zip(comps, ocomps):
+        if not hasattr(c, "_waveform"):
+            continue
+        for q in range(0, 400, 7):
+            w = c._waveform(q)
+            if o.kind == "point":
+                if o.pulse:
+                    h = o._pulse_value(q)
+                    ref = o.amplitude * h if h is not None else 0.0
+                else:
+                    ref = o.amplitude * o._sine(q)
+            elif o.kind == "line":
+                h = o._pulse_value(q) if o.pulse else o._sine(q)
+                ref = h if h is not None else 0.0
+            else:
+                ref = o._sine(q)
+            assert w == ref
+
+
+def test_pml_profiles_closed_form():
+    bE, cE, bH, cH, sE, sH = pml_profiles(4, False, 1e-8, 1.0, 0.5)
+    v = np.array([3.5, 2.5, 1.5, 0.5])
+    np.testing.assert_allclose(sE, 40 * v ** 3 / 625, rtol=1e-15)
+    np.testing.assert_allclose(bE, np.exp(-(sE + 1e-8) * 0.5), rtol=1e-15)
+    assert sH[-1] == 0 and cH[-1] == 0
+    bE2, _, _, _, sE2, _ = pml_profiles(4, True, 1e-8, 1.0, 0.5)
+    np.testing.assert_array_equal(sE2, sE[::-1])
+
+
+# ---- reference-style API tests (tests/test_grid.py, test_boundaries.py, test_sources.py)
+def test_grid_shape_of_ints_and_floats():
+    assert fdtd.Grid(shape=(3, 3, 3)).shape == (3, 3, 3)
+    assert fdtd.Grid(shape=(10.0e-9, 10.0e-9, 10.0e-9), grid_spacing=5.0e-9).shape == (2, 2, 2)
+    assert fdtd.Grid(shape=(10.0e-9, 10.0e-9, 3), grid_spacing=5.0e-9).shape == (2, 2, 3)
+
+
+def test_default_courant_numbers():
+    assert fdtd.Grid(shape=(3, 1, 1)).courant_number == pytest.approx(1.0, rel=0.02)
+    assert fdtd.Grid(shape=(3, 3, 1)).courant_number == pytest.approx(0.5 ** 0.5, rel=0.02)
+    assert fdtd.Grid(shape=(3, 3, 3)).courant_number == pytest.approx((1 / 3) ** 0.5, rel=0.02)
+    with pytest.raises(ValueError):
+        fdtd.Grid(shape=(3, 3, 3), courant_number=0.6)
+
+
+def _grid():
+    return fdtd.Grid(shape=(10, 10, 10), grid_spacing=100e-9)
+
+
+def test_boundary_registration_rules():
+    grid = _grid()
+    pb = fdtd.PeriodicBoundary(name="periodic_boundary")
+    grid[0, :, :] = pb
+    assert pb in grid.boundaries and type(pb).__name__ == "_PeriodicBoundaryX"
+    with pytest.raises(IndexError):
+        _grid()[0:2, :, :] = fdtd.PeriodicBoundary()
+    with pytest.raises(IndexError):
+        _grid()[2, :, :] = fdtd.PeriodicBoundary()
+    grid = _grid()
+    pml = fdtd.PML(name="PML")
+    grid[0:3, :, :] = pml
+    assert pml in grid.boundaries and pml.thickness == 3
+    with pytest.raises(IndexError):
+        _grid()[2:4, :, :] = fdtd.PML()
+    with pytest.raises(AttributeError):
+        grid[0:2, :, :] = fdtd.PML()
+    with pytest.raises(ValueError):
+        grid[-3:, :, :] = fdtd.PML(name="PML")
+
+
+def test_domain_border_pml():
+    grid = _grid()
+    with pytest.raises(IndexError):
+        DomainBorderPML(grid, grid.Nx // 2 + 1)
+    DomainBorderPML(_grid(), 3)
+    grid = _grid()
+    DomainBorderPML(grid, 3)
+    assert sorted(type(b).__name__ for b in grid.boundaries) == sorted(
+        ["_PMLZlow", "_PMLZhigh", "_PMLXlow", "_PMLXhigh", "_PMLYlow", "_PMLYhigh"])
+
+
+def test_plane_source_polarization():
+    with pytest.raises(ValueError):
+        _grid()[0, :, :] = fdtd.PlaneSource(polarization="x")
+    with pytest.raises(ValueError):
+        _grid()[:, 0, :] = fdtd.PlaneSource(polarization="y")
+    with pytest.raises(ValueError):
+        _grid()[:, :, 0] = fdtd.PlaneSource(polarization="z")
+    table = [((0, slice(None), slice(None)), "y", 1, 2), ((0, slice(None), slice(None)), "z", 2, 1),
+             ((slice(None), 0, slice(None)), "x", 0, 2), ((slice(None), 0, slice(None)), "z", 2, 0),
+             ((slice(None), slice(None), 0), "x", 0, 1), ((slice(None), slice(None), 0), "y", 1, 0)]
+    for key, pol, e, h in table:
+        ps = fdtd.PlaneSource(polarization=pol)
+        _grid()[key] = ps
+        assert (ps._Epol, ps._Hpol) == (e, h)
+
+
+def test_point_and_line_source_errors():
+    with pytest.raises(ValueError):
+        _grid()[1:3, 2, 2] = fdtd.PointSource()
+    with pytest.raises(ValueError):
+        _grid()[2, 2, 2:3] = fdtd.LineSource()
+
+
+def test_set_backend_names():
+    fdtd.set_backend("hip")
+    assert fdtd.backend.float == torch.float64
+    fdtd.set_backend("hip.float32")
+    assert fdtd.backend.float == torch.float32
+    fdtd.set_backend("hip.float64")
+    with pytest.raises(ValueError):
+        fdtd.set_backend("numpy")
+    with pytest.raises(ValueError):
+        fdtd.set_backend("hip.float16")
+
+
+def test_stepping_refuses_host_arrays():
+    grid = _grid()
+    with pytest.raises(RuntimeError):
+        grid.step()
