@@ -70,6 +70,20 @@ def algorithmic_bytes_per_half_step(n, pml, itemsize):
     return cells * 12 * itemsize + 6 * slab * 4 * itemsize
 
 
+def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
+    """Plain streaming device copy of `nbytes` (libfdtd_hip's k_copy): the
+    practical HBM bandwidth next to the 8 TB/s peak, and the reference for
+    scaling byte counters."""
+    from fdtd_amd import _native
+
+    src = torch.ones(nbytes // 8, dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    stream = torch.cuda.current_stream().cuda_stream
+    _native.bench_copy(dst, src, nbytes, 2, stream)
+    ms = _native.bench_copy(dst, src, nbytes, reps, stream)
+    return 2 * nbytes * reps / (ms / 1e3) / 1e9
+
+
 def cpu_baseline(n, pml, steps):
     """The numpy oracle (single-threaded ufuncs) on the same workload."""
     from oracle import fdtd_oracle as orc
@@ -136,6 +150,8 @@ def main():
     dom = max(per_launch, key=per_launch.get)
     achieved = half_bytes / (per_launch[dom] / 1e3) / 1e9
 
+    copy_gbs = copy_bandwidth(torch)
+
     base = None
     if not args.no_cpu_baseline:
         base = cpu_baseline(n, pml, args.cpu_steps)
@@ -157,6 +173,7 @@ def main():
         "config": {"workload": f"{n}^3 grid, {pml}-cell PML x6, 24 dielectric spheres, pulsed point source, {dt}",
                    "grid": [n, n, n], "pml": pml, "parallelism": "single GPU"},
         "step_hbm_roofline_frac": round(step_frac, 4),
+        "copy_GBps": round(copy_gbs, 1),
         "roofline": {"bound": "hbm", "kernel": f"{dom}<{'double' if dt == 'f64' else 'float'}>",
                      "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,

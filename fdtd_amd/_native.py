@@ -38,7 +38,15 @@ _SIGNATURES = {
     "fdtd_half_step_H": ([_vp, _c_i64, _vp], _c_int),
     "fdtd_run": ([_vp, _c_i64, _c_i64, _vp], _c_int),
     "fdtd_bench_main_kernel": ([_vp, _c_int, _c_i64, _vp, ctypes.POINTER(ctypes.c_float)], _c_int),
+    "fdtd_bench_copy": ([_vp, _vp, _c_i64, _c_i64, _vp, ctypes.POINTER(ctypes.c_float)], _c_int),
 }
+
+
+def bench_copy(dst, src, nbytes, reps, stream):
+    """Milliseconds for `reps` plain device copies of nbytes (measurement hook)."""
+    ms = ctypes.c_float()
+    check(lib().fdtd_bench_copy(dst.data_ptr(), src.data_ptr(), int(nbytes), int(reps), stream, ctypes.byref(ms)))
+    return ms.value
 
 
 def exported_symbols():

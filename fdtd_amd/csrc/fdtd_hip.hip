@@ -14,9 +14,10 @@
 // numpy reference bit for bit.
 //
 // Kernels
-//   k_main_E / k_main_H  one thread per cell: curl, material update, objects
-//                        and every CPML face registered before the first
-//                        periodic boundary. Memory-bound (~0.16 flop/B).
+//   k_main_E / k_main_H  curl, material update, objects and every CPML face
+//                        registered before the first periodic boundary; a
+//                        workgroup marches along x over a (y,z) tile.
+//                        Memory-bound (~0.16 flop/B).
 //   k_face_E / k_face_H  one CPML face over its slab, for faces registered
 //                        after a periodic boundary (order matters there).
 //   k_periodic           plane copy of a periodic boundary.
@@ -25,6 +26,8 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <algorithm>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -96,78 +99,80 @@ struct MainArgs {
   const ObjDev* objs;
   int32_t nobjs;
   int32_t bz_shift;  // block extent along z is 1 << bz_shift
+  int32_t cx;        // planes per workgroup along x
 };
 
 // ----------------------------------------------------------- device helpers
-__device__ __forceinline__ int face_local(const FaceDev& f, const Dims& d, int i, int j, int k) {
-  const int g = f.axis == 0 ? i : (f.axis == 1 ? j : k);
-  const int n = f.axis == 0 ? d.nx : (f.axis == 1 ? d.ny : d.nz);
-  return f.high ? g - (n - f.t) : g;
-}
-
-__device__ __forceinline__ int64_t face_offset(const FaceDev& f, const Dims& d, int i, int j, int k, int l) {
-  if (f.axis == 0) return ((int64_t)l * d.ny + j) * d.nz + k;
-  if (f.axis == 1) return ((int64_t)i * f.t + l) * d.nz + k;
-  return ((int64_t)i * d.ny + j) * f.t + l;
-}
-
-__device__ __forceinline__ int64_t axis_stride(const Dims& d, int axis) {
-  return axis == 0 ? (int64_t)d.ny * d.nz : (axis == 1 ? (int64_t)d.nz : 1);
-}
-
-// The CPML face's two live components: the field components other than the
-// normal axis, in increasing order. The convolution term of component c is
-// +psi when the axis follows c cyclically, -psi otherwise
-// (boundaries.py:457-459, 485-487 with only the normal-axis psi non-zero).
-__device__ __forceinline__ int face_comp(int axis, int q) {
-  return q == 0 ? (axis == 0 ? 1 : 0) : (axis == 2 ? 1 : 2);
-}
-
-// E-side psi update of one face cell plus the E correction
-// (boundaries.py:433-459 then :409-419). `kc` holds S*inv_eps per component.
+// The six one-sided differences of one cell, named after the curl terms:
+// zy = difference of component z along y, etc. (grid.py:29-76).
 template <typename T>
-__device__ __forceinline__ void face_update_E(const FaceDev& f, const Dims& d, const T* __restrict__ H,
-                                              int64_t cell, int l, int64_t off, const T kc[3], T e[3]) {
-  T* psi = reinterpret_cast<T*>(f.psiE);
-  const T b = (T)f.bE[l];
-  const T c = (T)f.cE[l];
-  const int64_t st = axis_stride(d, f.axis);
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int comp = face_comp(f.axis, q);
-    const T* Hs = H + (int64_t)(3 - f.axis - comp) * d.cs;
-    T p = psi[q * f.slab + off] * b;
-    if (l >= 1) {
-      const T dh = Hs[cell] - Hs[cell - st];
-      p = p + dh * c;
-    }
-    psi[q * f.slab + off] = p;
-    const T corr = kc[comp] * p;
-    e[comp] = (f.axis == (comp + 1) % 3) ? e[comp] + corr : e[comp] - corr;
+struct Diffs {
+  T zy, yz, xz, zx, yx, xy;
+};
+
+// One CPML face at one cell (boundaries.py:409-487). A face normal to axis
+// `ax` keeps psi for the two field components other than ax, C0 < C1
+// (C0 = ax==0 ? 1 : 0, C1 = ax==2 ? 1 : 2). psi_q <- psi_q*b, plus
+// (difference along ax of the other field's component 3-ax-C_q) * c where
+// the slab index allows it (E side: l >= 1; H side: l <= t-2). The
+// convolution term of C_q is +psi when ax follows C_q cyclically, -psi
+// otherwise, so E[C0] gets +-(S inv)*psi_0 and E[C1] +-(S inv)*psi_1 (H with
+// the opposite sign). Everything that depends on the axis is chosen with
+// selects rather than branches, so no address is formed from values merged
+// across control flow. Returns without touching v when the cell is outside
+// the face's slab.
+template <bool IS_E, typename T>
+__device__ __forceinline__ void pml_face(const FaceDev& fd, const Dims& d, int i, int j, int k, T zx, T yx, T zy,
+                                         T xy, T yz, T xz, T k0, T k1, T k2, T& v0, T& v1, T& v2) {
+  const int ax = fd.axis;
+  const int g = ax == 0 ? i : (ax == 1 ? j : k);
+  const int n = ax == 0 ? d.nx : (ax == 1 ? d.ny : d.nz);
+  const int l = fd.high ? g - (n - fd.t) : g;
+  if (l < 0 || l >= fd.t) return;
+  // slab: the grid with the normal axis shortened to t
+  const int64_t sny = ax == 1 ? fd.t : d.ny;
+  const int64_t snz = ax == 2 ? fd.t : d.nz;
+  const int64_t li = ax == 0 ? l : i, lj = ax == 1 ? l : j, lk = ax == 2 ? l : k;
+  const int64_t off = (li * sny + lj) * snz + lk;
+  T* __restrict__ psi = reinterpret_cast<T*>(IS_E ? fd.psiE : fd.psiH);
+  const double* bt = IS_E ? fd.bE : fd.bH;
+  const double* ct = IS_E ? fd.cE : fd.cH;
+  const T b = (T)bt[l], c = (T)ct[l];
+  const T d0 = ax == 0 ? zx : (ax == 1 ? zy : yz);
+  const T d1 = ax == 0 ? yx : (ax == 1 ? xy : xz);
+  T p0 = psi[off] * b;
+  T p1 = psi[fd.slab + off] * b;
+  if (IS_E ? (l >= 1) : (l <= fd.t - 2)) {
+    p0 = p0 + d0 * c;
+    p1 = p1 + d1 * c;
   }
+  psi[off] = p0;
+  psi[fd.slab + off] = p1;
+  const T r0 = (ax == 0 ? k1 : k0) * p0;
+  const T r1 = (ax == 2 ? k1 : k2) * p1;
+  T n0, n1, n2;
+  if (IS_E) {
+    // ax 0: E1 -= r0, E2 += r1; ax 1: E0 += r0, E2 -= r1; ax 2: E0 -= r0, E1 += r1
+    n0 = ax == 1 ? v0 + r0 : (ax == 2 ? v0 - r0 : v0);
+    n1 = ax == 0 ? v1 - r0 : (ax == 2 ? v1 + r1 : v1);
+    n2 = ax == 0 ? v2 + r1 : (ax == 1 ? v2 - r1 : v2);
+  } else {
+    n0 = ax == 1 ? v0 - r0 : (ax == 2 ? v0 + r0 : v0);
+    n1 = ax == 0 ? v1 + r0 : (ax == 2 ? v1 - r1 : v1);
+    n2 = ax == 0 ? v2 - r1 : (ax == 1 ? v2 + r1 : v2);
+  }
+  v0 = n0;
+  v1 = n1;
+  v2 = n2;
 }
 
-// H-side psi update plus the H correction (boundaries.py:461-487, :421-431).
-template <typename T>
-__device__ __forceinline__ void face_update_H(const FaceDev& f, const Dims& d, const T* __restrict__ E,
-                                              int64_t cell, int l, int64_t off, const T kc[3], T h[3]) {
-  T* psi = reinterpret_cast<T*>(f.psiH);
-  const T b = (T)f.bH[l];
-  const T c = (T)f.cH[l];
-  const int64_t st = axis_stride(d, f.axis);
-#pragma unroll
-  for (int q = 0; q < 2; ++q) {
-    const int comp = face_comp(f.axis, q);
-    const T* Es = E + (int64_t)(3 - f.axis - comp) * d.cs;
-    T p = psi[q * f.slab + off] * b;
-    if (l <= f.t - 2) {
-      const T de = Es[cell + st] - Es[cell];
-      p = p + de * c;
-    }
-    psi[q * f.slab + off] = p;
-    const T corr = kc[comp] * p;
-    h[comp] = (f.axis == (comp + 1) % 3) ? h[comp] - corr : h[comp] + corr;
-  }
+// All faces in `faces` that contain cell (i,j,k), in registration order.
+template <bool IS_E, typename T>
+__device__ __forceinline__ void pml_cell(const FaceDev* __restrict__ faces, int nfaces, const Dims& d, int i, int j,
+                                         int k, const Diffs<T>& df, const T kc[3], T v[3]) {
+  for (int fi = 0; fi < nfaces; ++fi)
+    pml_face<IS_E>(faces[fi], d, i, j, k, df.zx, df.yx, df.zy, df.xy, df.yz, df.xz, kc[0], kc[1], kc[2], v[0], v[1],
+                   v[2]);
 }
 
 // Object updates of one cell, registration order (objects.py:118-269).
@@ -202,125 +207,213 @@ __device__ __forceinline__ void objects_update_E(const ObjDev* __restrict__ objs
   }
 }
 
+// Logical tile of hardware workgroup w: workgroups are dealt round-robin to
+// the 8 XCDs (each with its own L2), so consecutive logical tiles -- which
+// share halo planes -- are given to the same XCD.
+__device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned nb) {
+  const unsigned per = nb / 8, rem = nb % 8, x = w % 8, idx = w / 8;
+  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
 // ------------------------------------------------------------------ kernels
-template <typename T>
+// Main E kernel. A workgroup owns a (y, z) tile of 256 columns (z fastest, one
+// row of bz cells per wavefront when bz = 64) and marches along x over a chunk
+// of `cx` planes: the x-neighbours come from registers of the previous plane,
+// the z-neighbours from the neighbouring lane, the y-neighbours from cache.
+template <typename T, bool OBJS>
 __global__ void __launch_bounds__(256) k_main_E(MainArgs<T> a) {
-  const int bz = 1 << a.bz_shift;
-  const int k = blockIdx.x * bz + (threadIdx.x & (bz - 1));
-  const int j = blockIdx.y * (256 >> a.bz_shift) + (threadIdx.x >> a.bz_shift);
-  const int i = blockIdx.z;
   const Dims d = a.d;
-  if (k >= d.nz || j >= d.ny) return;
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = 256 >> bzs;
+  const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  const int lz = threadIdx.x & (bz - 1);
+  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
+  const bool act = k < d.nz && j < d.ny;
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
-  const int64_t cell = i * sx + j * sy + k;
   const T* __restrict__ Hx = a.G;
   const T* __restrict__ Hy = a.G + cs;
   const T* __restrict__ Hz = a.G + 2 * cs;
-  const T hx = Hx[cell], hy = Hy[cell], hz = Hz[cell];
+  T* __restrict__ F = a.F;
+  const T* __restrict__ M = a.mat;
   const T zero = T(0);
-  // backward differences; an absent one contributes nothing (grid.py:67-74)
-  const T dHz_y = j > 0 ? hz - Hz[cell - sy] : zero;
-  const T dHy_z = k > 0 ? hy - Hy[cell - 1] : zero;
-  const T dHx_z = k > 0 ? hx - Hx[cell - 1] : zero;
-  const T dHz_x = i > 0 ? hz - Hz[cell - sx] : zero;
-  const T dHy_x = i > 0 ? hy - Hy[cell - sx] : zero;
-  const T dHx_y = j > 0 ? hx - Hx[cell - sy] : zero;
-  T curl[3];
-  curl[0] = (zero + dHz_y) - dHy_z;
-  curl[1] = (zero + dHx_z) - dHz_x;
-  curl[2] = (zero + dHy_x) - dHx_y;
-  T kc[3], e[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    kc[c] = a.sc * a.mat[c * cs + cell];
-    e[c] = a.F[c * cs + cell] + kc[c] * curl[c];
+  int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
+  T hz_prev = zero, hy_prev = zero;
+  if (act && i0 > 0) {
+    hz_prev = Hz[cell - sx];
+    hy_prev = Hy[cell - sx];
   }
-  if (a.nobjs) objects_update_E(a.objs, a.nobjs, a.sc, i, j, k, curl, e);
-  for (int f = 0; f < a.nfaces; ++f) {
-    const FaceDev& fd = a.faces[f];
-    const int l = face_local(fd, d, i, j, k);
-    if (l < 0 || l >= fd.t) continue;
-    face_update_E(fd, d, a.G, cell, l, face_offset(fd, d, i, j, k, l), kc, e);
-  }
+  for (int i = i0; i < i1; ++i, cell += sx) {
+    T hx = zero, hy = zero, hz = zero;
+    if (act) {
+      hx = Hx[cell];
+      hy = Hy[cell];
+      hz = Hz[cell];
+    }
+    T hy_km = __shfl_up(hy, 1);
+    T hx_km = __shfl_up(hx, 1);
+    if (act) {
+      if (lz == 0 && k > 0) {
+        hy_km = Hy[cell - 1];
+        hx_km = Hx[cell - 1];
+      }
+      T hz_jm = zero, hx_jm = zero;
+      if (j > 0) {
+        hz_jm = Hz[cell - sy];
+        hx_jm = Hx[cell - sy];
+      }
+      // backward differences; an absent one contributes nothing (grid.py:67-74)
+      Diffs<T> df;
+      df.zy = j > 0 ? hz - hz_jm : zero;
+      df.yz = k > 0 ? hy - hy_km : zero;
+      df.xz = k > 0 ? hx - hx_km : zero;
+      df.zx = i > 0 ? hz - hz_prev : zero;
+      df.yx = i > 0 ? hy - hy_prev : zero;
+      df.xy = j > 0 ? hx - hx_jm : zero;
+      T curl[3];
+      curl[0] = (zero + df.zy) - df.yz;
+      curl[1] = (zero + df.xz) - df.zx;
+      curl[2] = (zero + df.yx) - df.xy;
+      T kc[3], e[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) a.F[c * cs + cell] = e[c];
+      for (int c = 0; c < 3; ++c) {
+        kc[c] = a.sc * M[c * cs + cell];
+        e[c] = F[c * cs + cell] + kc[c] * curl[c];
+      }
+      if (OBJS) objects_update_E(a.objs, a.nobjs, a.sc, i, j, k, curl, e);
+      if (a.nfaces) pml_cell<true>(a.faces, a.nfaces, d, i, j, k, df, kc, e);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[c * cs + cell] = e[c];
+    }
+    hz_prev = hz;
+    hy_prev = hy;
+  }
 }
 
+// Main H kernel: the mirror image of the E kernel. Its differences are
+// forward ones, so workgroups take the tiles in decreasing order (the +y/+z
+// neighbour tiles were read just before, by the same XCD) and march along -x,
+// keeping E of plane i+1 in registers.
 template <typename T>
 __global__ void __launch_bounds__(256) k_main_H(MainArgs<T> a) {
-  const int bz = 1 << a.bz_shift;
-  const int k = blockIdx.x * bz + (threadIdx.x & (bz - 1));
-  const int j = blockIdx.y * (256 >> a.bz_shift) + (threadIdx.x >> a.bz_shift);
-  const int i = blockIdx.z;
   const Dims d = a.d;
-  if (k >= d.nz || j >= d.ny) return;
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = 256 >> bzs;
+  const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
+  const unsigned tile = gridDim.x - 1 - xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  const int lz = threadIdx.x & (bz - 1);
+  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
+  const bool act = k < d.nz && j < d.ny;
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
-  const int64_t cell = i * sx + j * sy + k;
   const T* __restrict__ Ex = a.G;
   const T* __restrict__ Ey = a.G + cs;
   const T* __restrict__ Ez = a.G + 2 * cs;
-  const T ex = Ex[cell], ey = Ey[cell], ez = Ez[cell];
+  T* __restrict__ F = a.F;
+  const T* __restrict__ M = a.mat;
   const T zero = T(0);
-  // forward differences; an absent one contributes nothing (grid.py:42-49)
-  const T dEz_y = j < d.ny - 1 ? Ez[cell + sy] - ez : zero;
-  const T dEy_z = k < d.nz - 1 ? Ey[cell + 1] - ey : zero;
-  const T dEx_z = k < d.nz - 1 ? Ex[cell + 1] - ex : zero;
-  const T dEz_x = i < d.nx - 1 ? Ez[cell + sx] - ez : zero;
-  const T dEy_x = i < d.nx - 1 ? Ey[cell + sx] - ey : zero;
-  const T dEx_y = j < d.ny - 1 ? Ex[cell + sy] - ex : zero;
-  T curl[3];
-  curl[0] = (zero + dEz_y) - dEy_z;
-  curl[1] = (zero + dEx_z) - dEz_x;
-  curl[2] = (zero + dEy_x) - dEx_y;
-  T kc[3], h[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) {
-    kc[c] = a.sc * a.mat[c * cs + cell];
-    h[c] = a.F[c * cs + cell] - kc[c] * curl[c];
+  int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
+  T ez_next = zero, ey_next = zero;  // E at plane i+1
+  if (act && i1 < d.nx) {
+    ez_next = Ez[cell + sx];
+    ey_next = Ey[cell + sx];
   }
-  // objects leave H unchanged (objects.py:131-137, 223-229, 271-277)
-  for (int f = 0; f < a.nfaces; ++f) {
-    const FaceDev& fd = a.faces[f];
-    const int l = face_local(fd, d, i, j, k);
-    if (l < 0 || l >= fd.t) continue;
-    face_update_H(fd, d, a.G, cell, l, face_offset(fd, d, i, j, k, l), kc, h);
-  }
+  for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
+    T ex = zero, ey = zero, ez = zero;
+    if (act) {
+      ex = Ex[cell];
+      ey = Ey[cell];
+      ez = Ez[cell];
+    }
+    T ey_kp = __shfl_down(ey, 1);
+    T ex_kp = __shfl_down(ex, 1);
+    if (act) {
+      if (lz == bz - 1 && k < d.nz - 1) {
+        ey_kp = Ey[cell + 1];
+        ex_kp = Ex[cell + 1];
+      }
+      T ez_jp = zero, ex_jp = zero;
+      if (j < d.ny - 1) {
+        ez_jp = Ez[cell + sy];
+        ex_jp = Ex[cell + sy];
+      }
+      // forward differences; an absent one contributes nothing (grid.py:42-49)
+      Diffs<T> df;
+      df.zy = j < d.ny - 1 ? ez_jp - ez : zero;
+      df.yz = k < d.nz - 1 ? ey_kp - ey : zero;
+      df.xz = k < d.nz - 1 ? ex_kp - ex : zero;
+      df.zx = i < d.nx - 1 ? ez_next - ez : zero;
+      df.yx = i < d.nx - 1 ? ey_next - ey : zero;
+      df.xy = j < d.ny - 1 ? ex_jp - ex : zero;
+      T curl[3];
+      curl[0] = (zero + df.zy) - df.yz;
+      curl[1] = (zero + df.xz) - df.zx;
+      curl[2] = (zero + df.yx) - df.xy;
+      T kc[3], h[3];
 #pragma unroll
-  for (int c = 0; c < 3; ++c) a.F[c * cs + cell] = h[c];
+      for (int c = 0; c < 3; ++c) {
+        kc[c] = a.sc * M[c * cs + cell];
+        h[c] = F[c * cs + cell] - kc[c] * curl[c];
+      }
+      // objects leave H unchanged (objects.py:131-137, 223-229, 271-277)
+      if (a.nfaces) pml_cell<false>(a.faces, a.nfaces, d, i, j, k, df, kc, h);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) F[c * cs + cell] = h[c];
+    }
+    ez_next = ez;
+    ey_next = ey;
+  }
+}
+
+// Plain device copy of n 16-byte words: the bandwidth reference for the
+// step kernels (a stream that reads and writes every byte once).
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
 // One CPML face applied on its own (faces registered after a periodic
-// boundary): F is the field being updated, G the other one.
-template <typename T, bool IS_E>
+// boundary): F is the field being updated, G the other one. AX, the face's
+// normal axis, is a template parameter so that no index depends on a runtime
+// branch over the axis.
+template <typename T, bool IS_E, int AX>
 __global__ void __launch_bounds__(256) k_face(Dims d, T sc, T* F, const T* G, const T* mat, FaceDev fd) {
-  const int64_t n = fd.slab;
   const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= n) return;
-  int i, j, k, l;
-  if (fd.axis == 0) {
-    k = p % d.nz; j = (p / d.nz) % d.ny; l = p / ((int64_t)d.nz * d.ny);
-    i = fd.high ? d.nx - fd.t + l : l;
-  } else if (fd.axis == 1) {
-    k = p % d.nz; l = (p / d.nz) % fd.t; i = p / ((int64_t)d.nz * fd.t);
-    j = fd.high ? d.ny - fd.t + l : l;
-  } else {
-    l = p % fd.t; j = (p / fd.t) % d.ny; i = p / ((int64_t)fd.t * d.ny);
-    k = fd.high ? d.nz - fd.t + l : l;
-  }
+  if (p >= fd.slab) return;
+  // slab-local coordinates, the slab laid out like the grid with axis AX of length t
+  const int64_t snx = AX == 0 ? fd.t : d.nx, sny = AX == 1 ? fd.t : d.ny, snz = AX == 2 ? fd.t : d.nz;
+  const int64_t li = p / (sny * snz), lj = (p / snz) % sny, lk = p % snz;
+  (void)snx;
+  const int l = (int)(AX == 0 ? li : (AX == 1 ? lj : lk));
+  const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
+  const int g = fd.high ? n - fd.t + l : l;
+  const int i = AX == 0 ? g : (int)li, j = AX == 1 ? g : (int)lj, k = AX == 2 ? g : (int)lk;
   const int64_t cell = ((int64_t)i * d.ny + j) * d.nz + k;
+  const int64_t st = AX == 0 ? (int64_t)d.ny * d.nz : (AX == 1 ? (int64_t)d.nz : 1);
+  // the other field's components differenced by this face: 3-AX-C0, 3-AX-C1
+  constexpr int S0 = 3 - AX - (AX == 0 ? 1 : 0), S1 = 3 - AX - (AX == 2 ? 1 : 2);
+  const T* G0 = G + S0 * d.cs;
+  const T* G1 = G + S1 * d.cs;
+  T d0 = T(0), d1 = T(0);
+  if (IS_E && l >= 1) {
+    d0 = G0[cell] - G0[cell - st];
+    d1 = G1[cell] - G1[cell - st];
+  }
+  if (!IS_E && l <= fd.t - 2) {
+    d0 = G0[cell + st] - G0[cell];
+    d1 = G1[cell + st] - G1[cell];
+  }
   T kc[3], v[3];
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     kc[c] = sc * mat[c * d.cs + cell];
     v[c] = F[c * d.cs + cell];
   }
-  if (IS_E)
-    face_update_E(fd, d, G, cell, l, p, kc, v);
-  else
-    face_update_H(fd, d, G, cell, l, p, kc, v);
-  const int c0 = face_comp(fd.axis, 0), c1 = face_comp(fd.axis, 1);
-  F[c0 * d.cs + cell] = v[c0];
-  F[c1 * d.cs + cell] = v[c1];
+  // pml_face reads the two normal-axis differences
+  pml_face<IS_E>(fd, d, i, j, k, d0, d1, d0, d1, d0, d1, kc[0], kc[1], kc[2], v[0], v[1], v[2]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) F[c * d.cs + cell] = v[c];
 }
 
 // Periodic plane copy: to_low ? plane 0 <- plane n-1 : plane n-1 <- plane 0.
@@ -459,6 +552,7 @@ struct FdtdPlan {
   int n_fused = 0;           // leading faces folded into the main kernels
   std::vector<Bnd> stages;   // boundaries applied after the main kernel
   int64_t nE = 0, nH = 0;    // half steps taken (detector slot clocks)
+  int cx = 1;                // x planes per workgroup of the main kernels
 };
 
 namespace {
@@ -504,17 +598,21 @@ MainArgs<T> main_args(const FdtdPlan* p, bool is_H) {
   int bzs = 0;
   while ((1 << bzs) < p->d.nz && bzs < 6) ++bzs;
   a.bz_shift = bzs;
+  a.cx = p->cx;
   return a;
 }
 
 template <typename T>
 void launch_main(const FdtdPlan* p, const MainArgs<T>& a, bool is_H, hipStream_t st) {
   const int bz = 1 << a.bz_shift, by = 256 >> a.bz_shift;
-  dim3 grid((p->d.nz + bz - 1) / bz, (p->d.ny + by - 1) / by, p->d.nx);
+  const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
+  dim3 grid(tiles * (unsigned)((p->d.nx + a.cx - 1) / a.cx));
   if (is_H)
     hipLaunchKernelGGL(k_main_H<T>, grid, dim3(256), 0, st, a);
+  else if (a.nobjs)
+    hipLaunchKernelGGL((k_main_E<T, true>), grid, dim3(256), 0, st, a);
   else
-    hipLaunchKernelGGL(k_main_E<T>, grid, dim3(256), 0, st, a);
+    hipLaunchKernelGGL((k_main_E<T, false>), grid, dim3(256), 0, st, a);
 }
 
 template <typename T>
@@ -531,11 +629,15 @@ int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
                          axis, is_H ? 0 : 1);
     } else {
       const FaceDev& fd = p->h_faces[b.index];
-      const unsigned nb = (unsigned)((fd.slab + 255) / 256);
-      if (is_H)
-        hipLaunchKernelGGL((k_face<T, false>), dim3(nb), dim3(256), 0, st, p->d, a.sc, a.F, a.G, a.mat, fd);
-      else
-        hipLaunchKernelGGL((k_face<T, true>), dim3(nb), dim3(256), 0, st, p->d, a.sc, a.F, a.G, a.mat, fd);
+      const dim3 nb((unsigned)((fd.slab + 255) / 256)), nt(256);
+      switch (fd.axis * 2 + (is_H ? 1 : 0)) {
+        case 0: hipLaunchKernelGGL((k_face<T, true, 0>), nb, nt, 0, st, p->d, a.sc, a.F, a.G, a.mat, fd); break;
+        case 1: hipLaunchKernelGGL((k_face<T, false, 0>), nb, nt, 0, st, p->d, a.sc, a.F, a.G, a.mat, fd); break;
+        case 2: hipLaunchKernelGGL((k_face<T, true, 1>), nb, nt, 0, st, p->d, a.sc, a.F, a.G, a.mat, fd); break;
+        case 3: hipLaunchKernelGGL((k_face<T, false, 1>), nb, nt, 0, st, p->d, a.sc, a.F, a.G, a.mat, fd); break;
+        case 4: hipLaunchKernelGGL((k_face<T, true, 2>), nb, nt, 0, st, p->d, a.sc, a.F, a.G, a.mat, fd); break;
+        default: hipLaunchKernelGGL((k_face<T, false, 2>), nb, nt, 0, st, p->d, a.sc, a.F, a.G, a.mat, fd); break;
+      }
     }
   }
   bool any_src = false;
@@ -739,6 +841,17 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     else
       p->stages.push_back(b);
   }
+  // x-chunk of the marching main kernels: enough workgroups to fill 256 CUs
+  // several times over, long enough chunks to amortise the halo plane
+  {
+    int bzs = 0;
+    while ((1 << bzs) < p->d.nz && bzs < 6) ++bzs;
+    const int bz = 1 << bzs, by = 256 >> bzs;
+    const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by);
+    int64_t cx = tiles * p->d.nx / 8192;
+    if (const char* env = getenv("FDTD_HIP_CX")) cx = atoi(env);
+    p->cx = (int)std::max<int64_t>(1, std::min<int64_t>(cx, p->d.nx));
+  }
   // byte layout of the table block
   const size_t nf = p->faces.size(), no = p->objs.size(), ns = p->srcs.size(), nd = p->dets.size();
   size_t off = 0;
@@ -889,6 +1002,32 @@ int fdtd_bench_main_kernel(FdtdPlan* p, int is_H, int64_t reps, void* stream, fl
     else
       launch_main<float>(p, main_args<float>(p, is_H != 0), is_H != 0, st);
   }
+  FDTD_HIP(hipGetLastError());
+  FDTD_HIP(hipEventRecord(e1, st));
+  FDTD_HIP(hipEventSynchronize(e1));
+  FDTD_HIP(hipEventElapsedTime(ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return 0;
+}
+
+int fdtd_bench_copy(void* dst, const void* src, int64_t bytes, int64_t reps, void* stream, float* ms) {
+  if (!dst || !src || bytes < 16 || (bytes & 15) || reps < 1 || !ms) return fail(-1, "bad copy benchmark arguments");
+  if ((reinterpret_cast<uintptr_t>(dst) | reinterpret_cast<uintptr_t>(src)) & 15)
+    return fail(-1, "copy buffers must be 16-byte aligned");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  int dev = 0, cus = 0;
+  FDTD_HIP(hipGetDevice(&dev));
+  FDTD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+  const int64_t n = bytes / 16;
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)cus * 16);
+  hipEvent_t e0, e1;
+  FDTD_HIP(hipEventCreate(&e0));
+  FDTD_HIP(hipEventCreate(&e1));
+  FDTD_HIP(hipEventRecord(e0, st));
+  for (int64_t r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(k_copy, dim3(blocks), dim3(256), 0, st, reinterpret_cast<const uint4*>(src),
+                       reinterpret_cast<uint4*>(dst), n);
   FDTD_HIP(hipGetLastError());
   FDTD_HIP(hipEventRecord(e1, st));
   FDTD_HIP(hipEventSynchronize(e1));

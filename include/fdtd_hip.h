@@ -122,6 +122,12 @@ int fdtd_run(FdtdPlan* plan, int64_t q0, int64_t nsteps, void* stream);
  * periodic stages, sources or detectors), so use it on a scratch state. */
 int fdtd_bench_main_kernel(FdtdPlan* plan, int is_H, int64_t reps, void* stream, float* ms);
 
+/* Measurement hook, no counterpart in the reference: copy `bytes` (a multiple
+ * of 16, both pointers 16-byte aligned) from src to dst `reps` times with a
+ * plain streaming kernel between two HIP events on `stream`; *ms receives the
+ * elapsed milliseconds. The practical HBM bandwidth next to the 8 TB/s peak. */
+int fdtd_bench_copy(void* dst, const void* src, int64_t bytes, int64_t reps, void* stream, float* ms);
+
 #ifdef __cplusplus
 }
 #endif
