@@ -102,6 +102,61 @@ def cpu_baseline(n, pml, steps):
                        f"{os.cpu_count()} host cores visible, ufuncs single-threaded")
 
 
+TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                               "k_main_256cube_pml16_hbm_traffic.md")
+
+
+def profiled_traffic(kernel, ctype, n, pml):
+    """HBM bytes per launch of `kernel` from the committed FETCH_SIZE /
+    WRITE_SIZE summary (counters scaled by a plain copy of known size), for
+    the configuration that summary was taken on; None otherwise."""
+    if (n, pml) != (256, 16) or not os.path.exists(TRAFFIC_PROFILE):
+        return None
+    for line in open(TRAFFIC_PROFILE):
+        cells = [c.strip().strip("`") for c in line.split("|")]
+        if len(cells) > 5 and cells[1].startswith(f"{kernel}<{ctype}"):
+            return int(float(cells[4]))
+    return None
+
+
+def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps):
+    """Mcells/s over `steps` timed steps and the roofline of the dominant kernel."""
+    fdtd.set_backend(f"hip.{dtype}")
+    grid = build_grid(fdtd, n, pml)
+    grid.run(warmup, progress_bar=False)
+    torch.cuda.synchronize()
+    stream = torch.cuda.current_stream()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    ev0.record(stream)
+    grid.run(steps, progress_bar=False)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    ms = ev0.elapsed_time(ev1)
+    ms_step = ms / steps
+    itemsize = 8 if dtype == "float64" else 4
+    half_bytes = algorithmic_bytes_per_half_step(n, pml, itemsize)
+    # dominant kernel: the fused main kernels alone, HIP events on the launch stream
+    plan = grid._ensure_plan()
+    plan.bench_main_kernel(5, stream.cuda_stream)
+    tE, tH = plan.bench_main_kernel(kernel_reps, stream.cuda_stream)
+    per_launch = {"k_main_E": tE, "k_main_H": tH}
+    dom = max(per_launch, key=per_launch.get)
+    achieved = half_bytes / (per_launch[dom] / 1e3) / 1e9
+    del grid, plan
+    torch.cuda.empty_cache()
+    return dict(
+        mcells=n ** 3 * steps / (ms / 1e3) / 1e6,
+        ms_step=ms_step,
+        step_frac=2 * half_bytes / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9),
+        roofline={"bound": "hbm", "kernel": f"{dom}<{'double' if itemsize == 8 else 'float'}>",
+                  "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                  "frac": round(achieved / HBM_PEAK_GBS, 4),
+                  "traffic": profiled_traffic(dom, "double" if itemsize == 8 else "float", n, pml),
+                  "algorithmic_bytes_per_launch": half_bytes,
+                  "avg_launch_ms": {k: round(v, 5) for k, v in per_launch.items()}})
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -113,6 +168,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
+    ap.add_argument("--no-fp32", action="store_true", help="skip the float32 measurement")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)\n")
@@ -122,34 +178,11 @@ def main():
 
     import fdtd_amd as fdtd
 
-    fdtd.set_backend(f"hip.{args.dtype}")
     n, pml = args.n, args.pml
-    grid = build_grid(fdtd, n, pml)
-    grid.run(args.warmup, progress_bar=False)
-    torch.cuda.synchronize()
-    stream = torch.cuda.current_stream()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    grid.run(args.steps, progress_bar=False)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
-    ms_step = ms / args.steps
-    mcells = n ** 3 * args.steps / (ms / 1e3) / 1e6
-    itemsize = 8 if args.dtype == "float64" else 4
-    half_bytes = algorithmic_bytes_per_half_step(n, pml, itemsize)
-    step_frac = 2 * half_bytes / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9)
-
-    # dominant kernel: time the fused main kernels alone (events on the launch stream)
-    plan = grid._ensure_plan()
-    per_launch = {}
-    for is_H, nm in ((0, "k_main_E"), (1, "k_main_H")):
-        plan.bench_main_kernel(is_H, 5, stream.cuda_stream)
-        per_launch[nm] = plan.bench_main_kernel(is_H, args.kernel_reps, stream.cuda_stream) / args.kernel_reps
-    dom = max(per_launch, key=per_launch.get)
-    achieved = half_bytes / (per_launch[dom] / 1e3) / 1e9
-
+    main = measure(torch, fdtd, args.dtype, n, pml, args.steps, args.warmup, args.kernel_reps)
+    other = None
+    if args.dtype == "float64" and not args.no_fp32:
+        other = measure(torch, fdtd, "float32", n, pml, args.steps, args.warmup, args.kernel_reps)
     copy_gbs = copy_bandwidth(torch)
 
     base = None
@@ -159,12 +192,12 @@ def main():
     dt = "f64" if args.dtype == "float64" else "f32"
     out = {
         "metric": METRIC,
-        "value": round(mcells, 1),
+        "value": round(main["mcells"], 1),
         "unit": "Mcells/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms_step, 4),
+        "ms_per_step": round(main["ms_step"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -172,15 +205,15 @@ def main():
         "data": "synthetic",
         "config": {"workload": f"{n}^3 grid, {pml}-cell PML x6, 24 dielectric spheres, pulsed point source, {dt}",
                    "grid": [n, n, n], "pml": pml, "parallelism": "single GPU"},
-        "step_hbm_roofline_frac": round(step_frac, 4),
+        "step_hbm_roofline_frac": round(main["step_frac"], 4),
         "copy_GBps": round(copy_gbs, 1),
-        "roofline": {"bound": "hbm", "kernel": f"{dom}<{'double' if dt == 'f64' else 'float'}>",
-                     "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                     "algorithmic_bytes_per_launch": half_bytes,
-                     "avg_launch_ms": {k: round(v, 5) for k, v in per_launch.items()}},
+        "roofline": main["roofline"],
         "cpu_baseline": base,
     }
+    if other is not None:
+        out["fp32"] = {"value": round(other["mcells"], 1), "unit": "Mcells/s", "ms_per_step": round(other["ms_step"], 4),
+                       "step_hbm_roofline_frac": round(other["step_frac"], 4), "roofline": other["roofline"],
+                       "speedup_vs_f64": round(other["mcells"] / main["mcells"], 3)}
     print(json.dumps(out), flush=True)
 
 

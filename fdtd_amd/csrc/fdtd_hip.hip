@@ -87,6 +87,8 @@ struct Dims {
   int64_t cs;  // component stride nx*ny*nz
 };
 
+constexpr int FOLDED_POINTS = 4;
+
 template <typename T>
 struct MainArgs {
   Dims d;
@@ -100,6 +102,14 @@ struct MainArgs {
   int32_t nobjs;
   int32_t bz_shift;  // block extent along z is 1 << bz_shift
   int32_t cx;        // planes per workgroup along x
+  int32_t by;        // block extent along y (threads per block = bz * by)
+  // point sources folded into the E kernel (no periodic stage, no detector):
+  // cell, and the waveform value to add at this step, in registration order
+  int32_t npts;
+  int64_t pt_cell[FOLDED_POINTS];
+  const double* wave;  // waveform table, row s = source s
+  int64_t wave_cap;
+  int64_t q;
 };
 
 // ----------------------------------------------------------- device helpers
@@ -207,6 +217,20 @@ __device__ __forceinline__ void objects_update_E(const ObjDev* __restrict__ objs
   }
 }
 
+// Loads and stores of the read-once streams (the updated field and the
+// material); NT marks them non-temporal so the L2 keeps the other field,
+// whose planes and rows are re-read as halos.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
 // Logical tile of hardware workgroup w: workgroups are dealt round-robin to
 // the 8 XCDs (each with its own L2), so consecutive logical tiles -- which
 // share halo planes -- are given to the same XCD.
@@ -220,10 +244,10 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned nb) {
 // row of bz cells per wavefront when bz = 64) and marches along x over a chunk
 // of `cx` planes: the x-neighbours come from registers of the previous plane,
 // the z-neighbours from the neighbouring lane, the y-neighbours from cache.
-template <typename T, bool OBJS>
-__global__ void __launch_bounds__(256) k_main_E(MainArgs<T> a) {
+template <typename T, bool OBJS, bool NT>
+__global__ void __launch_bounds__(1024) k_main_E(MainArgs<T> a) {
   const Dims d = a.d;
-  const int bzs = a.bz_shift, bz = 1 << bzs, by = 256 >> bzs;
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
   const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
@@ -278,13 +302,15 @@ __global__ void __launch_bounds__(256) k_main_E(MainArgs<T> a) {
       T kc[3], e[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        kc[c] = a.sc * M[c * cs + cell];
-        e[c] = F[c * cs + cell] + kc[c] * curl[c];
+        kc[c] = a.sc * ld_stream<NT>(M + c * cs + cell);
+        e[c] = ld_stream<NT>(F + c * cs + cell) + kc[c] * curl[c];
       }
       if (OBJS) objects_update_E(a.objs, a.nobjs, a.sc, i, j, k, curl, e);
       if (a.nfaces) pml_cell<true>(a.faces, a.nfaces, d, i, j, k, df, kc, e);
+      for (int s = 0; s < a.npts; ++s)  // sources.py:109, registration order
+        if (a.pt_cell[s] == cell) e[2] = e[2] + (T)a.wave[s * a.wave_cap + a.q];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) F[c * cs + cell] = e[c];
+      for (int c = 0; c < 3; ++c) st_stream<NT>(F + c * cs + cell, e[c]);
     }
     hz_prev = hz;
     hy_prev = hy;
@@ -295,10 +321,10 @@ __global__ void __launch_bounds__(256) k_main_E(MainArgs<T> a) {
 // forward ones, so workgroups take the tiles in decreasing order (the +y/+z
 // neighbour tiles were read just before, by the same XCD) and march along -x,
 // keeping E of plane i+1 in registers.
-template <typename T>
-__global__ void __launch_bounds__(256) k_main_H(MainArgs<T> a) {
+template <typename T, bool NT>
+__global__ void __launch_bounds__(1024) k_main_H(MainArgs<T> a) {
   const Dims d = a.d;
-  const int bzs = a.bz_shift, bz = 1 << bzs, by = 256 >> bzs;
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
   const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
   const unsigned tile = gridDim.x - 1 - xcd_tile(blockIdx.x, gridDim.x);
   const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
@@ -353,13 +379,13 @@ __global__ void __launch_bounds__(256) k_main_H(MainArgs<T> a) {
       T kc[3], h[3];
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        kc[c] = a.sc * M[c * cs + cell];
-        h[c] = F[c * cs + cell] - kc[c] * curl[c];
+        kc[c] = a.sc * ld_stream<NT>(M + c * cs + cell);
+        h[c] = ld_stream<NT>(F + c * cs + cell) - kc[c] * curl[c];
       }
       // objects leave H unchanged (objects.py:131-137, 223-229, 271-277)
       if (a.nfaces) pml_cell<false>(a.faces, a.nfaces, d, i, j, k, df, kc, h);
 #pragma unroll
-      for (int c = 0; c < 3; ++c) F[c * cs + cell] = h[c];
+      for (int c = 0; c < 3; ++c) st_stream<NT>(F + c * cs + cell, h[c]);
     }
     ez_next = ez;
     ey_next = ey;
@@ -553,6 +579,9 @@ struct FdtdPlan {
   std::vector<Bnd> stages;   // boundaries applied after the main kernel
   int64_t nE = 0, nH = 0;    // half steps taken (detector slot clocks)
   int cx = 1;                // x planes per workgroup of the main kernels
+  int block_threads = 256;   // threads per workgroup of the main kernels
+  bool fold_points = false;  // point sources added inside the E kernel
+  bool nontemporal = true;   // read-once streams marked non-temporal
 };
 
 namespace {
@@ -599,25 +628,43 @@ MainArgs<T> main_args(const FdtdPlan* p, bool is_H) {
   while ((1 << bzs) < p->d.nz && bzs < 6) ++bzs;
   a.bz_shift = bzs;
   a.cx = p->cx;
+  a.by = std::max(1, p->block_threads >> bzs);
+  a.npts = 0;
   return a;
 }
 
 template <typename T>
 void launch_main(const FdtdPlan* p, const MainArgs<T>& a, bool is_H, hipStream_t st) {
-  const int bz = 1 << a.bz_shift, by = 256 >> a.bz_shift;
+  const int bz = 1 << a.bz_shift, by = a.by;
   const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
-  dim3 grid(tiles * (unsigned)((p->d.nx + a.cx - 1) / a.cx));
-  if (is_H)
-    hipLaunchKernelGGL(k_main_H<T>, grid, dim3(256), 0, st, a);
-  else if (a.nobjs)
-    hipLaunchKernelGGL((k_main_E<T, true>), grid, dim3(256), 0, st, a);
-  else
-    hipLaunchKernelGGL((k_main_E<T, false>), grid, dim3(256), 0, st, a);
+  const dim3 grid(tiles * (unsigned)((p->d.nx + a.cx - 1) / a.cx)), block(bz * by);
+  if (p->nontemporal) {
+    if (is_H)
+      hipLaunchKernelGGL((k_main_H<T, true>), grid, block, 0, st, a);
+    else if (a.nobjs)
+      hipLaunchKernelGGL((k_main_E<T, true, true>), grid, block, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_main_E<T, false, true>), grid, block, 0, st, a);
+  } else {
+    if (is_H)
+      hipLaunchKernelGGL((k_main_H<T, false>), grid, block, 0, st, a);
+    else if (a.nobjs)
+      hipLaunchKernelGGL((k_main_E<T, true, false>), grid, block, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_main_E<T, false, false>), grid, block, 0, st, a);
+  }
 }
 
 template <typename T>
 int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
-  const MainArgs<T> a = main_args<T>(p, is_H);
+  MainArgs<T> a = main_args<T>(p, is_H);
+  if (!is_H && p->fold_points) {
+    a.npts = (int32_t)p->srcs.size();
+    for (int s = 0; s < a.npts; ++s) a.pt_cell[s] = p->srcs[s].cells[0];
+    a.wave = p->wave;
+    a.wave_cap = p->wave_cap;
+    a.q = q;
+  }
   launch_main<T>(p, a, is_H, st);
   for (const Bnd& b : p->stages) {
     if (b.periodic) {
@@ -641,7 +688,8 @@ int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
     }
   }
   bool any_src = false;
-  for (const SrcHost& s : p->srcs) any_src |= (!is_H || s.kind == FDTD_SRC_PLANE);
+  if (!p->fold_points)
+    for (const SrcHost& s : p->srcs) any_src |= (!is_H || s.kind == FDTD_SRC_PLANE);
   if (any_src || !p->dets.empty()) {
     TailArgs<T> t{};
     t.d = p->d;
@@ -841,17 +889,28 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     else
       p->stages.push_back(b);
   }
-  // x-chunk of the marching main kernels: enough workgroups to fill 256 CUs
-  // several times over, long enough chunks to amortise the halo plane
+  // launch shape of the main kernels: workgroup size, then the x-chunk --
+  // enough workgroups to fill 256 CUs many times over, long enough chunks to
+  // amortise the halo plane
   {
+    if (const char* env = getenv("FDTD_HIP_BLOCK")) {
+      const int bt = atoi(env);
+      if (bt == 256 || bt == 512 || bt == 1024) p->block_threads = bt;
+    }
     int bzs = 0;
     while ((1 << bzs) < p->d.nz && bzs < 6) ++bzs;
-    const int bz = 1 << bzs, by = 256 >> bzs;
+    const int bz = 1 << bzs, by = std::max(1, p->block_threads >> bzs);
     const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by);
-    int64_t cx = tiles * p->d.nx / 8192;
+    int64_t cx = tiles * p->d.nx / 32768;
     if (const char* env = getenv("FDTD_HIP_CX")) cx = atoi(env);
+    if (const char* env = getenv("FDTD_HIP_NT")) p->nontemporal = atoi(env) != 0;
     p->cx = (int)std::max<int64_t>(1, std::min<int64_t>(cx, p->d.nx));
   }
+  // point sources fold into the E kernel when nothing runs between the main
+  // kernel and the sources (no periodic or staged face) and no detector samples
+  p->fold_points = !p->srcs.empty() && (int)p->srcs.size() <= FOLDED_POINTS && p->dets.empty();
+  for (const SrcHost& s : p->srcs) p->fold_points = p->fold_points && s.kind == FDTD_SRC_POINT;
+  for (const Bnd& b : p->order) p->fold_points = p->fold_points && !b.periodic;
   // byte layout of the table block
   const size_t nf = p->faces.size(), no = p->objs.size(), ns = p->srcs.size(), nd = p->dets.size();
   size_t off = 0;
@@ -987,27 +1046,36 @@ int fdtd_run(FdtdPlan* p, int64_t q0, int64_t nsteps, void* stream) {
   return 0;
 }
 
-int fdtd_bench_main_kernel(FdtdPlan* p, int is_H, int64_t reps, void* stream, float* ms) {
+int fdtd_bench_main_kernel(FdtdPlan* p, int64_t reps, void* stream, float* ms_E, float* ms_H) {
   if (int rc = check_plan(p)) return rc;
-  if (reps < 1 || !ms) return fail(-1, "bad benchmark arguments");
+  if (reps < 1 || reps > 4096 || !ms_E || !ms_H) return fail(-1, "bad benchmark arguments");
   if (int rc = select_device(p)) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
-  hipEvent_t e0, e1;
-  FDTD_HIP(hipEventCreate(&e0));
-  FDTD_HIP(hipEventCreate(&e1));
-  FDTD_HIP(hipEventRecord(e0, st));
+  std::vector<hipEvent_t> ev(2 * reps + 1);
+  for (auto& e : ev) FDTD_HIP(hipEventCreate(&e));
+  FDTD_HIP(hipEventRecord(ev[0], st));
   for (int64_t r = 0; r < reps; ++r) {
-    if (p->dtype == FDTD_F64)
-      launch_main<double>(p, main_args<double>(p, is_H != 0), is_H != 0, st);
-    else
-      launch_main<float>(p, main_args<float>(p, is_H != 0), is_H != 0, st);
+    for (int h = 0; h < 2; ++h) {
+      if (p->dtype == FDTD_F64)
+        launch_main<double>(p, main_args<double>(p, h != 0), h != 0, st);
+      else
+        launch_main<float>(p, main_args<float>(p, h != 0), h != 0, st);
+      FDTD_HIP(hipEventRecord(ev[2 * r + h + 1], st));
+    }
   }
   FDTD_HIP(hipGetLastError());
-  FDTD_HIP(hipEventRecord(e1, st));
-  FDTD_HIP(hipEventSynchronize(e1));
-  FDTD_HIP(hipEventElapsedTime(ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
+  FDTD_HIP(hipEventSynchronize(ev[2 * reps]));
+  double tE = 0, tH = 0;
+  for (int64_t r = 0; r < reps; ++r) {
+    float a = 0, b = 0;
+    FDTD_HIP(hipEventElapsedTime(&a, ev[2 * r], ev[2 * r + 1]));
+    FDTD_HIP(hipEventElapsedTime(&b, ev[2 * r + 1], ev[2 * r + 2]));
+    tE += a;
+    tH += b;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *ms_E = (float)(tE / reps);
+  *ms_H = (float)(tH / reps);
   return 0;
 }
 
@@ -1020,7 +1088,8 @@ int fdtd_bench_copy(void* dst, const void* src, int64_t bytes, int64_t reps, voi
   FDTD_HIP(hipGetDevice(&dev));
   FDTD_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
   const int64_t n = bytes / 16;
-  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)cus * 16);
+  // 4 workgroups per CU: the fastest shape measured for a plain streaming copy
+  const unsigned blocks = (unsigned)std::min<int64_t>((n + 255) / 256, (int64_t)cus * 4);
   hipEvent_t e0, e1;
   FDTD_HIP(hipEventCreate(&e0));
   FDTD_HIP(hipEventCreate(&e1));

@@ -115,12 +115,13 @@ int fdtd_half_step_H(FdtdPlan* plan, int64_t q, void* stream);
 int fdtd_run(FdtdPlan* plan, int64_t q0, int64_t nsteps, void* stream);
 
 /* Measurement hook, no counterpart in the reference (used by bench.py to
- * price the dominant kernel against the HBM roofline): launch the fused main
- * kernel of one half step (is_H 0: E, 1: H) `reps` times back to back between
- * two HIP events recorded on `stream`, wait for the end event and store the
- * elapsed milliseconds in *ms. The launches advance the fields (without
- * periodic stages, sources or detectors), so use it on a scratch state. */
-int fdtd_bench_main_kernel(FdtdPlan* plan, int is_H, int64_t reps, void* stream, float* ms);
+ * price the dominant kernels against the HBM roofline): launch the fused main
+ * kernels of a step, E then H, `reps` times (at most 4096) on `stream` with a
+ * HIP event recorded after each launch, wait for the last one and store the
+ * average milliseconds of the E and of the H launches in *ms_E and *ms_H.
+ * The launches advance the fields (without periodic stages, sources or
+ * detectors), so use it on a scratch state. */
+int fdtd_bench_main_kernel(FdtdPlan* plan, int64_t reps, void* stream, float* ms_E, float* ms_H);
 
 /* Measurement hook, no counterpart in the reference: copy `bytes` (a multiple
  * of 16, both pointers 16-byte aligned) from src to dst `reps` times with a

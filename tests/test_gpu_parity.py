@@ -220,3 +220,22 @@ def test_float32_error_against_float64_reference(golden, name):
     eH = rel_l2(_np(grid.H), g["H"])
     _say(f"[gpu] float32 {name}: relative L2 error E {eE:.2e} H {eH:.2e}")
     assert eE < 1e-4 and eH < 1e-4
+
+
+@pytest.mark.parametrize("seed", range(16, 28))
+def test_float64_folded_point_sources_match_oracle(seed):
+    """Scenarios without detectors, line or plane sources: point sources are
+    then added inside the E kernel (when no periodic boundary follows)."""
+    sc = scenarios.random_scenario(seed)
+    sc["components"] = [c for c in sc["components"]
+                        if c[0] not in ("LineDetector", "BlockDetector", "LineSource", "PlaneSource")]
+    sc["components"].append(("PointSource", dict(period=7, amplitude=0.3), (1, 1, 0), {}))
+    _say(f"[gpu] folded-source scenario {seed}: shape {sc['shape']}, {[c[0] for c in sc['components']]}")
+    sim, _ = scenarios.build_oracle(orc, sc)
+    scenarios.run_program(sim, sc)
+    grid, _ = _build(sc)
+    scenarios.run_program(grid, sc)
+    torch.cuda.synchronize()
+    bitwise = not any(k[0] == "AnisotropicObject" for k in sc["components"])
+    _assert_same(_np(grid.E), sim.E, "E", bitwise)
+    _assert_same(_np(grid.H), sim.H, "H", bitwise)
