@@ -157,6 +157,149 @@ def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps):
                   "avg_launch_ms": {k: round(v, 5) for k, v in per_launch.items()}})
 
 
+def spd_inverse_block(shape, seed=7, lo=1.0, hi=9.0):
+    """Per-cell inverse of a random SPD 3x3 permittivity with eigenvalues in
+    [lo, hi]: R diag(1/lambda) R^T with R a random rotation (unit quaternion)."""
+    rng = np.random.default_rng(seed)
+    n = int(np.prod(shape))
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    w, x, y, z = q.T
+    R = np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)], -1),
+        np.stack([2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)], -1),
+        np.stack([2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)], -1)], -2)
+    lam = rng.uniform(lo, hi, size=(n, 3))
+    inv = np.einsum("nij,nj,nkj->nik", R, 1.0 / lam, R)
+    return inv.reshape(tuple(shape) + (3, 3))
+
+
+def build_precision_grid(fdtd, shape=(512, 512, 384), pml=20):
+    """The precision configuration: 20-cell PML on all faces, an absorbing
+    slab (conductivity 1e4 S/m), an anisotropic block with a seeded SPD
+    permittivity (eigenvalues 1-9) and a continuous 1550 nm point source."""
+    import torch
+
+    nx, ny, nz = shape
+    grid = fdtd.Grid(shape, grid_spacing=50e-9, courant_number=0.99 * 3 ** -0.5)
+    for k in PML_KEYS(pml):
+        grid[k] = fdtd.PML()
+    grid[int(0.65 * nx):int(0.65 * nx) + 24, pml:ny - pml, pml:nz - pml] = fdtd.AbsorbingObject(
+        permittivity=2.25, conductivity=1e4)
+    box = (slice(int(0.25 * nx), int(0.25 * nx) + 64), slice(int(0.4 * ny), int(0.4 * ny) + 64),
+           slice(int(0.35 * nz), int(0.35 * nz) + 48))
+    aniso = fdtd.AnisotropicObject(permittivity=1.0)
+    grid[box] = aniso
+    aniso.inverse_permittivity = torch.from_numpy(spd_inverse_block((64, 64, 48)))
+    grid[nx // 2, ny // 2, nz // 2] = fdtd.PointSource(period=1550e-9 / 299792458.0)
+    return grid
+
+
+def precision_form(args):
+    """float32 (and compensated float32) against the float64 path on the
+    precision configuration: throughput and field error after `steps`."""
+    import torch
+
+    import fdtd_amd as fdtd
+
+    shape = tuple(args.shape) if args.shape else (512, 512, 384)
+    steps = args.steps
+    out = {"form": "precision", "config": {"workload": "precision: 20-cell PML, absorbing slab, SPD anisotropic block",
+                                           "grid": list(shape), "steps": steps}, "runs": {}}
+    ref = None
+    for name in ["float64", "float32", "float32c"]:
+        fdtd.set_backend(f"hip.{name}")
+        grid = build_precision_grid(fdtd, shape)
+        grid.run(1, progress_bar=False)
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        grid.run(steps - 1, progress_bar=False)
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / (steps - 1)
+        run = {"ms_per_step": round(ms, 4), "Mcells/s": round(np.prod(shape) / (ms / 1e3) / 1e6, 1)}
+        fields = torch.cat([grid.E.reshape(-1), grid.H.reshape(-1)]).double()
+        if ref is None:
+            ref = fields
+        else:
+            diff = fields - ref
+            run["rel_l2_error_vs_f64"] = float(diff.norm() / ref.norm())
+            run["max_rel_error_vs_f64"] = float(diff.abs().max() / ref.abs().max())
+        out["runs"][name] = run
+        del grid, fields
+        torch.cuda.empty_cache()
+    f64 = out["runs"]["float64"]["ms_per_step"]
+    for name, run in out["runs"].items():
+        run["speedup_vs_f64_general_kernel"] = round(f64 / run["ms_per_step"], 3)
+    print(json.dumps(out), flush=True)
+
+
+def adjoint_measure(dtype="float32", checkpoints=("sqrt", "record"), reps=3):
+    """Gradient cost on the differentiation configuration (128x128x64, 12-cell
+    PML, 64x64x16 design Object with seeded eps in [1, 12], pulsed 1550 nm
+    line source, J = time-integrated |E|^2 over a 16x16 plane, 2000 steps):
+    wall time of one gradient evaluation (forward with records/checkpoints,
+    recomputation, transposed steps, all host work) over the wall time of the
+    plain forward run of the same steps; the best of `reps` of each."""
+    import torch
+
+    import fdtd_amd as fdtd
+    from fdtd_amd.adjoint import IntensityGradient
+
+    sys.path.insert(0, os.path.join(os.path.dirname(os.path.abspath(__file__)), "tests", "golden"))
+    import adjoint_scenarios as A
+
+    sc = A.C5
+    fdtd.set_backend(f"hip.{dtype}")
+    perm = torch.from_numpy(A.design_permittivity(sc))
+    grid, design = A.build(fdtd, sc, perm)
+    T = sc["steps"]
+
+    def timed(fn):
+        best = None
+        for _ in range(reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            fn()
+            torch.cuda.synchronize()
+            dt_ = time.perf_counter() - t0
+            best = dt_ if best is None else min(best, dt_)
+        return best * 1e3
+
+    probs = {c: IntensityGradient(grid, design, sc["plane"], T, c) for c in checkpoints}
+    first = probs[checkpoints[0]]
+    first._set_permittivity(perm)
+
+    def forward():
+        first._reset()
+        grid.run(T, progress_bar=False)
+
+    forward()
+    fwd_ms = timed(forward)
+    out = {"metric": "adjoint/forward time ratio", "unit": "x", "dtype": "f32" if dtype == "float32" else "f64",
+           "config": {"workload": "differentiation: 128x128x64, 12-cell PML x6, 64x64x16 design Object, "
+                                  "pulsed line source, sum_t sum_16x16 plane |E|^2", "steps": T},
+           "forward_ms": round(fwd_ms, 2), "gradient_ms": {}, "ratio": {}}
+    for c, prob in probs.items():
+        prob.gradient(perm)
+        ms = timed(lambda: prob.gradient(perm))
+        out["gradient_ms"][c] = round(ms, 2)
+        out["ratio"][c] = round(ms / fwd_ms, 3)
+        out.setdefault("segments", {})[c] = prob.last_info
+    out["value"] = out["ratio"][checkpoints[0]]
+    del grid, design, probs, first
+    torch.cuda.empty_cache()
+    return out
+
+
+def adjoint_form(args):
+    out = adjoint_measure("float64" if args.adjoint_f64 else "float32")
+    out["higher_is_better"] = False
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -169,10 +312,21 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-fp32", action="store_true", help="skip the float32 measurement")
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "adjoint"],
+                    help="plain: the 256^3 headline line; precision: float32 forms vs float64; "
+                         "adjoint: gradient/forward time on the differentiation configuration")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement of the plain line")
+    ap.add_argument("--adjoint-f64", action="store_true", help="--form adjoint in float64")
+    ap.add_argument("--shape", type=int, nargs=3, default=None, help="grid shape for --form precision")
     args = ap.parse_args()
     if args.gpus != 1:
         sys.stderr.write("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)\n")
         sys.exit(2)
+
+    if args.form == "precision":
+        return precision_form(args)
+    if args.form == "adjoint":
+        return adjoint_form(args)
 
     import torch
 
@@ -214,6 +368,8 @@ def main():
         out["fp32"] = {"value": round(other["mcells"], 1), "unit": "Mcells/s", "ms_per_step": round(other["ms_step"], 4),
                        "step_hbm_roofline_frac": round(other["step_frac"], 4), "roofline": other["roofline"],
                        "speedup_vs_f64": round(other["mcells"] / main["mcells"], 3)}
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_measure("float32")
     print(json.dumps(out), flush=True)
 
 

@@ -20,3 +20,4 @@ from .objects import Object, AbsorbingObject, AnisotropicObject
 from .boundaries import PeriodicBoundary, PML, DomainBorderPML
 from .backend import backend
 from .backend import set_backend
+from .backend import backend_names_available

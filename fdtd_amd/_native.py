@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
 
-F64, F32 = 0, 1
+F64, F32, F32C = 0, 1, 2
 OBJ_PLAIN, OBJ_ABSORBING, OBJ_ANISOTROPIC = 0, 1, 2
 SRC_POINT, SRC_LINE, SRC_PLANE = 0, 1, 2
 
@@ -20,12 +20,15 @@ _lib = None
 _c_int, _c_i64, _c_dbl, _vp = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
 _pi64 = ctypes.POINTER(ctypes.c_int64)
 _pdbl = ctypes.POINTER(ctypes.c_double)
+_pi32 = ctypes.POINTER(ctypes.c_int32)
 
 _SIGNATURES = {
     "fdtd_version": ([], ctypes.c_char_p),
     "fdtd_last_error": ([], ctypes.c_char_p),
     "fdtd_plan_create": ([_c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)], _c_int),
     "fdtd_plan_destroy": ([_vp], _c_int),
+    "fdtd_plan_set_compensation": ([_vp, _vp, _vp], _c_int),
+    "fdtd_plan_set_material_palette": ([_vp, _vp, _vp, _vp, _vp], _c_int),
     "fdtd_plan_add_pml": ([_vp, _c_int, _c_int, _c_int, _pdbl, _pdbl, _pdbl, _pdbl, _vp, _vp], _c_int),
     "fdtd_plan_add_periodic": ([_vp, _c_int], _c_int),
     "fdtd_plan_add_object": ([_vp, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _c_int, _vp, _vp], _c_int),
@@ -40,6 +43,11 @@ _SIGNATURES = {
     "fdtd_bench_main_kernel": ([_vp, _c_i64, _vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
                                _c_int),
     "fdtd_bench_copy": ([_vp, _vp, _c_i64, _c_i64, _vp, ctypes.POINTER(ctypes.c_float)], _c_int),
+    "fdtd_adjoint_psi_size": ([_vp, _pi64], _c_int),
+    "fdtd_adjoint_create": ([_vp, _vp, _vp, _vp, _vp, _vp, _pi32, _pi32, _vp, ctypes.POINTER(_vp)], _c_int),
+    "fdtd_adjoint_destroy": ([_vp], _c_int),
+    "fdtd_run_recorded": ([_vp, _c_i64, _c_i64, _vp, _vp, _vp], _c_int),
+    "fdtd_adjoint_run": ([_vp, _c_i64, _vp, _vp, _vp], _c_int),
 }
 
 
@@ -124,6 +132,15 @@ class Plan:
             self._L.fdtd_plan_destroy(h)
             self._h = None
 
+    def set_compensation(self, E_lo, H_lo):
+        check(self._L.fdtd_plan_set_compensation(self._h, E_lo.data_ptr(), H_lo.data_ptr()))
+        self._keep += [E_lo, H_lo]
+
+    def set_material_palette(self, eps_id, eps_tab, mu_id, mu_tab):
+        check(self._L.fdtd_plan_set_material_palette(self._h, eps_id.data_ptr(), eps_tab.data_ptr(),
+                                                     mu_id.data_ptr(), mu_tab.data_ptr()))
+        self._keep += [eps_id, eps_tab, mu_id, mu_tab]
+
     def add_pml(self, axis, high, thickness, bE, cE, bH, cH, psiE, psiH):
         check(self._L.fdtd_plan_add_pml(self._h, axis, int(high), thickness, _dbl_array(bE), _dbl_array(cE),
                                         _dbl_array(bH), _dbl_array(cH), psiE.data_ptr(), psiH.data_ptr()))
@@ -175,3 +192,41 @@ class Plan:
         e, h = ctypes.c_float(), ctypes.c_float()
         check(self._L.fdtd_bench_main_kernel(self._h, reps, stream, ctypes.byref(e), ctypes.byref(h)))
         return e.value, h.value
+
+
+class Adjoint:
+    """Owner of one FdtdAdjoint* over a Plan (reverse-mode steps for the
+    objective sum_steps sum_box |E|^2); the arrays are torch tensors kept
+    alive here."""
+
+    def __init__(self, plan, gcoef, dcoef, lamE, lamH, psi_adj, design_box, objective_box, grad):
+        L = plan._L
+        self._L = L
+        h = _vp()
+        db = (ctypes.c_int32 * 6)(*[int(v) for v in design_box])
+        ob = (ctypes.c_int32 * 6)(*[int(v) for v in objective_box])
+        check(L.fdtd_adjoint_create(plan._h, gcoef.data_ptr(), dcoef.data_ptr() if dcoef is not None else None,
+                                    lamE.data_ptr(), lamH.data_ptr(),
+                                    psi_adj.data_ptr() if psi_adj is not None and psi_adj.numel() else None,
+                                    db, ob, grad.data_ptr(), ctypes.byref(h)))
+        self._h = h
+        self._keep = [plan, gcoef, dcoef, lamE, lamH, psi_adj, grad]
+
+    @staticmethod
+    def psi_size(plan):
+        n = ctypes.c_int64()
+        check(plan._L.fdtd_adjoint_psi_size(plan._h, ctypes.byref(n)))
+        return n.value
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.fdtd_adjoint_destroy(h)
+            self._h = None
+
+    def run_recorded(self, q0, nsteps, curl_records, plane_records, stream):
+        check(self._L.fdtd_run_recorded(self._h, q0, nsteps, curl_records.data_ptr(), plane_records.data_ptr(),
+                                        stream))
+
+    def run_adjoint(self, nsteps, curl_records, plane_records, stream):
+        check(self._L.fdtd_adjoint_run(self._h, nsteps, curl_records.data_ptr(), plane_records.data_ptr(), stream))

@@ -1,0 +1,295 @@
+"""Reverse-mode gradients of HIP-backend runs with respect to a design region.
+
+The reference differentiates a run by stepping its torch backend with
+autograd enabled (fdtd/backend.py:207-361): torch records every array
+operation of Grid.update_E/update_H (fdtd/grid.py:275-325), PML.update_phi_*
+/update_* (fdtd/boundaries.py:409-487) and Object.update_E (fdtd/objects.py:
+118-129), then replays their transposes. Here the same derivative comes from
+hand-written transposed half-step kernels (libfdtd_hip: fdtd_run_recorded,
+fdtd_adjoint_run) for one objective,
+
+    J = sum over the T steps of sum over an objective box of |E|^2,
+
+with E taken after each step, and one design region: a plain ``Object``
+registered in the grid whose permittivity is the variable. The run starts from
+rest (E = H = CPML state = 0, time index 0), as a fresh reference grid does.
+
+Memory. The gradient needs, for every step, curl H inside the design box and
+E inside the objective box. ``checkpoint="record"`` keeps them for all T
+steps (one forward pass plus one reverse pass). ``checkpoint="sqrt"`` keeps
+a copy of the whole state (E, H, every CPML psi) every K = ceil(sqrt(T))
+steps instead; the reverse pass restores each checkpoint, steps its segment
+forward again recording K steps, and runs the transposed steps over it (two
+forward passes plus one reverse pass, O(sqrt(T)) states of memory).
+``"auto"`` picks "record" when the records fit in a quarter of the free
+device memory.
+"""
+
+import math
+
+import numpy as np
+import torch
+
+from . import _native
+from .boundaries import PML
+from .objects import AbsorbingObject, AnisotropicObject, Object
+
+
+def _box_of(grid, key):
+    """Half-open box (x0, x1, y0, y1, z0, z1) of a grid index tuple of ints
+    and unit-step slices."""
+    if not isinstance(key, tuple) or len(key) != 3:
+        raise IndexError("the objective box is indexed as grid[x, y, z] with ints or unit-step slices")
+    out = []
+    for k, n in zip(key, (grid.Nx, grid.Ny, grid.Nz)):
+        if isinstance(k, (int, np.integer)):
+            k = int(k)
+            if k < 0:
+                k += n
+            lo, hi = k, k + 1
+        elif isinstance(k, slice):
+            if k.step not in (None, 1):
+                raise IndexError("the objective box needs unit-step slices")
+            lo, hi, _ = k.indices(n)
+        else:
+            raise IndexError(f"invalid index {k!r} for the objective box")
+        if not 0 <= lo < hi <= n:
+            raise IndexError("empty or out-of-range objective box")
+        out += [lo, hi]
+    return tuple(out)
+
+
+class IntensityGradient:
+    """Objective J = sum_t sum_box |E|^2 of a T-step run from rest and its
+    gradient with respect to the permittivity of the design object.
+
+    grid:        a HIP-backend grid with PML faces, plain or absorbing
+                 objects and sources (no periodic boundaries, no anisotropic
+                 objects, float64 or float32 storage); detectors are ignored
+    design:      a plain ``Object`` registered in ``grid``
+    box:         the objective box, a grid index tuple (x, y, z)
+    steps:       T
+    checkpoint:  "sqrt" | "record" | "auto"
+    """
+
+    def __init__(self, grid, design, box, steps, checkpoint="sqrt"):
+        if grid.compensated:
+            raise NotImplementedError("gradients need hip.float64 or hip.float32 storage")
+        if type(design) is not Object or design.grid is not grid:
+            raise ValueError("the design region must be a plain Object registered in the grid")
+        if grid.boundaries and any(not isinstance(b, PML) for b in grid.boundaries):
+            raise NotImplementedError("gradients support PML boundaries only (no periodic boundaries)")
+        for obj in grid.objects:
+            if isinstance(obj, AnisotropicObject):
+                raise NotImplementedError("gradients do not support anisotropic objects")
+            if obj is not design and _overlap(obj, design):
+                raise ValueError("another object overlaps the design region")
+        if checkpoint not in ("sqrt", "record", "auto"):
+            raise ValueError("checkpoint must be 'sqrt', 'record' or 'auto'")
+        self.grid = grid
+        self.design = design
+        self.box = _box_of(grid, box)
+        self.steps = int(steps)
+        if self.steps < 1:
+            raise ValueError("steps must be positive")
+        d = design
+        self.design_box = (d.x.start, d.x.stop, d.y.start, d.y.stop, d.z.start, d.z.stop)
+        self.nd = d.Nx * d.Ny * d.Nz
+        b = self.box
+        self.np = (b[1] - b[0]) * (b[3] - b[2]) * (b[5] - b[4])
+        self.checkpoint = checkpoint
+        # the design's border entries hold the grid's values, not the
+        # permittivity's (objects.py:79-90): keep them
+        inv = design._device_array("inverse_permittivity", (d.Nx, d.Ny, d.Nz, 3))
+        self._border = torch.zeros(inv.shape, dtype=torch.bool, device=inv.device)
+        if d.Nx > 1:
+            self._border[-1, :, :, 0] = True
+        if d.Ny > 1:
+            self._border[:, -1, :, 1] = True
+        if d.Nz > 1:
+            self._border[:, :, -1, 2] = True
+        self._border_values = inv[self._border].clone()
+        self.last_info = {}
+
+    # ---------------------------------------------------------- plumbing
+    def _set_permittivity(self, permittivity):
+        """design.inverse_permittivity <- 1/permittivity in place (border
+        entries kept); returns the broadcast permittivity it used."""
+        d = self.design
+        inv = d._device_array("inverse_permittivity", (d.Nx, d.Ny, d.Nz, 3))
+        p = torch.as_tensor(permittivity).detach().to(device=inv.device, dtype=torch.float64)
+        if p.dim() == 3:
+            p = p[..., None]
+        p = p.expand(d.Nx, d.Ny, d.Nz, 3)
+        new = (1.0 / p).to(inv.dtype)
+        new[self._border] = self._border_values
+        inv.copy_(new)
+        d.permittivity = permittivity
+        return p
+
+    def _coefficients(self):
+        """(gcoef, dcoef) of the composed E update E' = dcoef E + gcoef curl H."""
+        g = self.grid
+        S = g.courant_number
+        gcoef = (S * g._inv_eps.to(torch.float64))
+        dcoef = None
+        for obj in g.objects:
+            sl = (slice(None), obj.x, obj.y, obj.z)
+            inv = obj.inverse_permittivity.to(torch.float64).permute(3, 0, 1, 2)
+            if isinstance(obj, AbsorbingObject):
+                f = obj.absorption_factor.to(torch.float64).permute(3, 0, 1, 2)
+                a = (1 - f) / (1 + f)
+                if dcoef is None:
+                    dcoef = torch.ones_like(gcoef)
+                dcoef[sl] = dcoef[sl] * a
+                gcoef[sl] = gcoef[sl] * a + S * inv / (1 + f)
+            else:
+                gcoef[sl] = gcoef[sl] + S * inv
+        dt = g.dtype
+        return gcoef.to(dt).contiguous(), (dcoef.to(dt).contiguous() if dcoef is not None else None)
+
+    def _plan(self):
+        """A plan of the grid's boundaries, objects and sources (no detectors)
+        with a waveform table covering the T steps, built once (the object
+        arrays it reads are updated in place)."""
+        if getattr(self, "_cached_plan", None) is None:
+            self._cached_plan = self._new_plan()
+        return self._cached_plan
+
+    def _new_plan(self):
+        g = self.grid
+        if g._E.device.type != "cuda":
+            raise RuntimeError("gradients need a HIP device")
+        code = _native.F64 if g.dtype == torch.float64 else _native.F32
+        plan = _native.Plan(code, (g.Nx, g.Ny, g.Nz), g.courant_number, g._E, g._H, g._inv_eps, g._inv_mu)
+        for comp in list(g.boundaries) + list(g.objects) + list(g.sources):
+            comp._add_to_plan(plan)
+        if g.sources:
+            table = np.empty((len(g.sources), self.steps), dtype=np.float64)
+            for r, src in enumerate(g.sources):
+                table[r] = [src._waveform(q) for q in range(self.steps)]
+            wave = torch.from_numpy(table).to(g._E.device)
+            plan.set_waveforms(wave)
+        else:
+            plan.set_waveforms(None)
+        plan.finalize()
+        return plan
+
+    def _state(self):
+        g = self.grid
+        out = [g._E, g._H]
+        for b in g.boundaries:
+            out += [b._psiE, b._psiH]
+        return out
+
+    def _reset(self):
+        for t in self._state():
+            t.zero_()
+        self.grid.time_steps_passed = 0
+
+    def _mode(self):
+        if self.checkpoint != "auto":
+            return self.checkpoint
+        esz = self.grid._E.element_size()
+        need = self.steps * 3 * (self.nd + self.np) * esz
+        free, _ = torch.cuda.mem_get_info(self.grid._E.device)
+        return "record" if need <= free // 4 else "sqrt"
+
+    # ------------------------------------------------------------- public
+    def objective(self, permittivity):
+        """J for the given design permittivity (forward only)."""
+        return self._run(permittivity, gradient=False)[0]
+
+    def gradient(self, permittivity):
+        """(J, dJ/d permittivity) for the given design permittivity; the
+        gradient has the shape of ``permittivity``, float64."""
+        return self._run(permittivity, gradient=True)
+
+    def _run(self, permittivity, gradient):
+        g = self.grid
+        perm_b = self._set_permittivity(permittivity)
+        plan = self._plan()
+        dev, dt = g._E.device, g.dtype
+        stream = torch.cuda.current_stream().cuda_stream
+        T = self.steps
+        mode = self._mode() if gradient else "sqrt"
+        K = T if mode == "record" else max(1, math.isqrt(T - 1) + 1)
+        gcoef, dcoef = self._coefficients()
+        lamE = torch.zeros_like(g._E)
+        lamH = torch.zeros_like(g._H)
+        psi = torch.zeros(_native.Adjoint.psi_size(plan), dtype=dt, device=dev)
+        grad = torch.zeros((3, self.nd), dtype=torch.float64, device=dev)
+        adj = _native.Adjoint(plan, gcoef, dcoef, lamE, lamH, psi, self.design_box, self.box, grad)
+        curl = torch.empty((K, 3, self.nd), dtype=dt, device=dev)
+        plane = torch.empty((K, 3, self.np), dtype=dt, device=dev)
+        J = torch.zeros((), dtype=torch.float64, device=dev)
+
+        self._reset()
+        segments = [(q, min(K, T - q)) for q in range(0, T, K)]
+        checkpoints = []
+        for q, m in segments:
+            if gradient and mode == "sqrt":
+                checkpoints.append([t.clone() for t in self._state()])
+            adj.run_recorded(q, m, curl, plane, stream)
+            J += plane[:m].to(torch.float64).square().sum()
+        g.time_steps_passed = T
+        self.last_info = {"mode": mode, "segment": K, "checkpoints": len(checkpoints)}
+        if not gradient:
+            return float(J), None
+        final = [t.clone() for t in self._state()] if mode == "sqrt" else None
+        for s in range(len(segments) - 1, -1, -1):
+            q, m = segments[s]
+            if mode == "sqrt":
+                for dst, src in zip(self._state(), checkpoints[s]):
+                    dst.copy_(src)
+                checkpoints[s] = None
+                adj.run_recorded(q, m, curl, plane, stream)
+            adj.run_adjoint(m, curl, plane, stream)
+        if final is not None:
+            for dst, src in zip(self._state(), final):
+                dst.copy_(src)
+        d = self.design
+        g_inv = grad.reshape(3, d.Nx, d.Ny, d.Nz).permute(1, 2, 3, 0).clone()
+        g_inv[self._border] = 0
+        # inv = 1/perm: dJ/dperm = -dJ/dinv / perm^2, summed over broadcast axes
+        g_perm_b = -g_inv / perm_b.square()
+        return float(J), _reduce_to(g_perm_b, permittivity)
+
+
+def _reduce_to(g, permittivity):
+    """Sum a (Nx, Ny, Nz, 3) gradient down to the shape of `permittivity`."""
+    shape = tuple(torch.as_tensor(permittivity).shape)
+    if len(shape) == 3:
+        return g.sum(dim=3)
+    if len(shape) == 4:
+        return g if shape[3] == 3 else g.sum(dim=3, keepdim=True)
+    if len(shape) == 0:
+        return g.sum()
+    raise ValueError(f"unsupported permittivity shape {shape}")
+
+
+def _overlap(a, b):
+    return all(max(sa.start, sb.start) < min(sa.stop, sb.stop) for sa, sb in ((a.x, b.x), (a.y, b.y), (a.z, b.z)))
+
+
+class _IntensityFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, permittivity, problem):
+        J, grad = problem.gradient(permittivity)
+        ctx.save_for_backward(grad.to(device=permittivity.device, dtype=permittivity.dtype))
+        return torch.tensor(J, dtype=torch.float64, device=permittivity.device)
+
+    @staticmethod
+    def backward(ctx, gJ):
+        (grad,) = ctx.saved_tensors
+        return gJ.to(grad.dtype) * grad, None
+
+
+def time_integrated_intensity(permittivity, grid, design, box, steps, checkpoint="sqrt"):
+    """J = sum over `steps` steps from rest of sum over `box` of |E|^2, as a
+    differentiable function of the design object's permittivity (a torch
+    tensor, typically with requires_grad=True); backward() yields dJ/d
+    permittivity from the transposed HIP steps."""
+    problem = IntensityGradient(grid, design, box, steps, checkpoint)
+    perm = permittivity if torch.is_tensor(permittivity) else torch.as_tensor(permittivity, dtype=torch.float64)
+    return _IntensityFunction.apply(perm, problem)

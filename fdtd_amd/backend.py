@@ -5,6 +5,9 @@ The package provides the HIP backends only:
     - ``hip``          float64 fields
     - ``hip.float64``
     - ``hip.float32``  float32 fields (all arithmetic in float32)
+    - ``hip.float32c`` float32 fields with an 8-bit compensation term per
+                       value (the float's rounding residual), float64
+                       arithmetic, materials and CPML state
 
 Arrays are torch tensors on the HIP device (torch names it "cuda" on ROCm);
 the time stepping runs in hand-written HIP kernels (libfdtd_hip.so). The
@@ -25,9 +28,10 @@ import torch
 backend_names = [
     dict(backends="hip.float64"),
     dict(backends="hip.float32"),
+    dict(backends="hip.float32c"),
 ]
 
-_DTYPES = {"float64": torch.float64, "float32": torch.float32}
+_DTYPES = {"float64": torch.float64, "float32": torch.float32, "float32c": torch.float32}
 _NP = {torch.float64: numpy.float64, torch.float32: numpy.float32}
 
 # Registration-only mode for hosts without a HIP device (the unit tests of the
@@ -40,6 +44,11 @@ def host_only(flag=True):
     exercised without a GPU; stepping then raises RuntimeError."""
     global _HOST_ONLY
     _HOST_ONLY = bool(flag)
+
+
+def backend_names_available():
+    """Backend names set_backend accepts."""
+    return ["hip"] + ["hip." + d for d in _DTYPES]
 
 
 def device_available():
@@ -61,6 +70,12 @@ class HipBackend(Backend):
     int = torch.int64
     float = torch.float64
     complex = torch.complex128
+    compensated = False
+
+    @property
+    def material_float(self):
+        """dtype of materials, object arrays and CPML state"""
+        return torch.float64 if self.compensated else self.float
 
     @property
     def device(self):
@@ -74,7 +89,8 @@ class HipBackend(Backend):
 
     @property
     def numpy_float(self):
-        return _NP[self.float]
+        """numpy dtype of host-side material / profile computations"""
+        return _NP[self.material_float]
 
     # -- constructors
     def _dtype(self, dtype):
@@ -170,7 +186,8 @@ backend = HipBackend()
 
 
 def set_backend(name: str):
-    """Select ``hip`` / ``hip.float64`` / ``hip.float32`` (fdtd/backend.py:367-439)."""
+    """Select ``hip`` / ``hip.float64`` / ``hip.float32`` / ``hip.float32c``
+    (fdtd/backend.py:367-439)."""
     if name.count(".") == 0:
         base, dtype = name, "float64"
     elif name.count(".") == 1:
@@ -182,9 +199,10 @@ def set_backend(name: str):
             f"Unknown backend '{name}'. fdtd_amd provides 'hip', 'hip.float64' and 'hip.float32'; "
             "the numpy/torch backends belong to the reference fdtd package")
     if dtype not in _DTYPES:
-        raise ValueError(f"Unknown dtype '{dtype}' for the HIP backend (float64 or float32)")
+        raise ValueError(f"Unknown dtype '{dtype}' for the HIP backend (float64, float32 or float32c)")
     if not _HOST_ONLY and not torch.cuda.is_available():
         raise RuntimeError(
             "HIP backend is not available.\nDo you have an AMD GPU visible to PyTorch-ROCm?")
     backend.__class__ = HipBackend
     backend.float = _DTYPES[dtype]
+    backend.compensated = dtype == "float32c"

@@ -194,8 +194,8 @@ class PML(Boundary):
         prof = pml_profiles(thickness, self.high, self.a, self.k, g.courant_number, bd.numpy_float)
         self._bE, self._cE, self._bH, self._cH, self._sigE, self._sigH = prof
         dev = g._E.device
-        self._psiE = torch.zeros((2,) + self.shape, dtype=g.dtype, device=dev)
-        self._psiH = torch.zeros((2,) + self.shape, dtype=g.dtype, device=dev)
+        self._psiE = torch.zeros((2,) + self.shape, dtype=g.mat_dtype, device=dev)
+        self._psiH = torch.zeros((2,) + self.shape, dtype=g.mat_dtype, device=dev)
         sl = [slice(None)] * 3
         sl[self.axis] = slice(-thickness, None) if self.high else slice(None, thickness)
         self.loc = tuple(sl) + (slice(None),)
@@ -244,14 +244,14 @@ class PML(Boundary):
         return self._full(self._sigH, 0.0)
 
     def _psi_full(self, store, comp):
-        out = torch.zeros(self.shape + (3,), dtype=self.grid.dtype, device=store.device)
+        out = torch.zeros(self.shape + (3,), dtype=store.dtype, device=store.device)
         comps = [c for c in range(3) if c != self.axis]
         if comp in comps:
             out[..., self.axis] = store[comps.index(comp)]
         return out
 
     def _phi(self, store):
-        out = torch.zeros(self.shape + (3,), dtype=self.grid.dtype, device=store.device)
+        out = torch.zeros(self.shape + (3,), dtype=store.dtype, device=store.device)
         for q, c in enumerate(c for c in range(3) if c != self.axis):
             out[..., c] = store[q] if self.axis == (c + 1) % 3 else -store[q]
         return out

@@ -33,7 +33,7 @@ def flags():
     return [
         f"--offload-arch={ARCH}",
         "-O3",
-        "-std=c++17",
+        "-std=c++20",
         "-fPIC",
         "-shared",
         "-ffp-contract=off",

@@ -1,0 +1,879 @@
+// Device code of libfdtd_hip: fused Yee half-step kernels for MI355X (gfx950).
+//
+// One E half step is, per cell and in this order (fdtd/grid.py:275-299):
+//   curl = backward-difference curl of H                 (grid.py:54-76)
+//   E   += (S * inv_eps) * curl                          (grid.py:283)
+//   objects in registration order                        (objects.py:118-269)
+//   boundaries in registration order: CPML correction    (boundaries.py:409-459)
+//     or periodic plane copy                             (boundaries.py:184-219)
+//   sources in registration order, then detectors        (sources.py, detectors.py)
+// and the H half step mirrors it (grid.py:301-325). Every floating-point
+// operation is the one the reference performs, in the same order, with
+// contraction disabled (-ffp-contract=off), so float64 results match the
+// numpy reference bit for bit.
+//
+// The kernels are written once over two policies:
+//   field storage  PlainField<S, R>  values stored as S, computed in R
+//                  CompField         float plus an 8-bit compensation term
+//                                    (the rounding residual of the float,
+//                                    scaled to its ulp), computed in double
+//   material       MatArray<S, R>    per-cell (3, N) array
+//                  MatPalette<R>     one byte per cell indexing <= 256 float64
+//                                    triples
+//
+// Kernels
+//   k_main_E / k_main_H  curl, material update, objects and every CPML face
+//                        registered before the first periodic boundary; a
+//                        workgroup marches along x over a (y,z) tile.
+//                        Memory-bound (~0.16 flop/B).
+//   k_face               one CPML face over its slab, for faces registered
+//                        after a periodic boundary (order matters there).
+//   k_periodic           plane copy of a periodic boundary.
+//   k_tail               one workgroup: sources (in order) then detectors.
+//   k_copy               plain streaming copy (bandwidth reference).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "fdtd_hip.h"
+
+namespace fdtd_dev {
+
+// ---------------------------------------------------------------- plan data
+struct FaceDev {
+  int32_t axis, high, t, pad;
+  const double* bE;
+  const double* cE;
+  const double* bH;
+  const double* cH;
+  void* psiE;  // psi in the arithmetic type R of the plan
+  void* psiH;
+  int64_t slab;  // cells in the face slab
+};
+
+struct ObjDev {
+  int32_t kind, x0, x1, y0, y1, z0, z1, pad;
+  const void* inv;     // in the plan's object type
+  const void* absorb;
+};
+
+struct SrcDev {
+  int32_t kind, n, compE, compH;
+  const int64_t* cells;
+  const double* profile;
+  double amplitude;
+};
+
+struct DetDev {
+  int32_t n, pad;
+  const int64_t* cells;
+  void* bufE;
+  void* bufH;
+  int64_t slot0E, slot0H;  // slot written at half-step counter c is c - slot0
+};
+
+struct Dims {
+  int32_t nx, ny, nz, pad;
+  int64_t cs;  // component stride nx*ny*nz
+};
+
+// ------------------------------------------------------- load/store helpers
+// Loads and stores of the read-once streams (the updated field and the
+// material); NT marks them non-temporal so the L2 keeps the other field,
+// whose planes and rows are re-read as halos.
+template <bool NT, typename T>
+__device__ __forceinline__ T ld_stream(const T* p) {
+  if constexpr (NT) return __builtin_nontemporal_load(p);
+  else return *p;
+}
+template <bool NT, typename T>
+__device__ __forceinline__ void st_stream(T* p, T v) {
+  if constexpr (NT) __builtin_nontemporal_store(v, p);
+  else *p = v;
+}
+
+// ------------------------------------------------------------ field storage
+template <typename S, typename R_>
+struct PlainField {
+  using R = R_;
+  using Sample = S;
+  S* p;
+  __device__ __forceinline__ R get(int64_t i) const { return (R)p[i]; }
+  template <bool NT>
+  __device__ __forceinline__ R get_s(int64_t i) const { return (R)ld_stream<NT>(p + i); }
+  template <bool NT>
+  __device__ __forceinline__ void put_s(int64_t i, R v) const { st_stream<NT>(p + i, (S)v); }
+  __device__ __forceinline__ void put(int64_t i, R v) const { p[i] = (S)v; }
+  __device__ __forceinline__ void copy(int64_t dst, int64_t src) const { p[dst] = p[src]; }
+  __device__ __forceinline__ Sample sample(int64_t i) const { return p[i]; }
+};
+
+// 2^e as a double, e in [-1022, 1023]
+__device__ __forceinline__ double pow2(int e) { return __longlong_as_double((long long)(e + 1023) << 52); }
+
+// Exponent E of a float in frexp's convention (h = m 2^E, m in [1/2, 1));
+// zero and subnormals use -125 so that their ulp is 2^(E-24) as well.
+__device__ __forceinline__ int comp_exp(float h) {
+  const int eb = (int)((__float_as_uint(h) >> 23) & 0xffu);
+  return eb == 0 ? -125 : eb - 126;
+}
+
+// A value v stored as h = (float)v and q = round((v - h) 2^(32-E)), |q| <= 127:
+// the residual |v - h| <= ulp(h)/2 = 2^(E-25) is kept to a quantum of 2^(E-32),
+// about 2^-31 of |h|. Both conversions are exact in double arithmetic.
+__device__ __forceinline__ double comp_decode(float h, int8_t q) {
+  return (double)h + (double)q * pow2(comp_exp(h) - 32);
+}
+__device__ __forceinline__ int8_t comp_encode(double v, float h) {
+  const double s = (v - (double)h) * pow2(32 - comp_exp(h));
+  return (int8_t)(int)fmin(127.0, fmax(-127.0, rint(s)));
+}
+
+struct CompField {
+  using R = double;
+  using Sample = float;
+  float* hi;
+  int8_t* lo;
+  __device__ __forceinline__ double get(int64_t i) const { return comp_decode(hi[i], lo[i]); }
+  template <bool NT>
+  __device__ __forceinline__ double get_s(int64_t i) const {
+    return comp_decode(ld_stream<NT>(hi + i), ld_stream<NT>(lo + i));
+  }
+  template <bool NT>
+  __device__ __forceinline__ void put_s(int64_t i, double v) const {
+    const float h = (float)v;
+    st_stream<NT>(hi + i, h);
+    st_stream<NT>(lo + i, comp_encode(v, h));
+  }
+  __device__ __forceinline__ void put(int64_t i, double v) const {
+    const float h = (float)v;
+    hi[i] = h;
+    lo[i] = comp_encode(v, h);
+  }
+  __device__ __forceinline__ void copy(int64_t dst, int64_t src) const {
+    hi[dst] = hi[src];
+    lo[dst] = lo[src];
+  }
+  __device__ __forceinline__ float sample(int64_t i) const { return hi[i]; }
+};
+
+// --------------------------------------------------------------- materials
+template <typename S, typename R>
+struct MatArray {
+  const S* p;  // (3, N)
+  int64_t cs;
+  template <bool NT>
+  __device__ __forceinline__ void get3(int64_t i, R m[3]) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[c] = (R)ld_stream<NT>(p + c * cs + i);
+  }
+};
+
+template <typename R>
+struct MatPalette {
+  const uint8_t* id;  // (N)
+  const double* tab;  // (256, 3)
+  template <bool NT>
+  __device__ __forceinline__ void get3(int64_t i, R m[3]) const {
+    const int k = (int)ld_stream<NT>(id + i);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) m[c] = (R)tab[3 * k + c];
+  }
+};
+
+// ------------------------------------------------------------ CPML helpers
+// The six one-sided differences of one cell, named after the curl terms:
+// zy = difference of component z along y, etc. (grid.py:29-76).
+template <typename T>
+struct Diffs {
+  T zy, yz, xz, zx, yx, xy;
+};
+
+// One CPML face at one cell (boundaries.py:409-487). A face normal to axis
+// `ax` keeps psi for the two field components other than ax, C0 < C1
+// (C0 = ax==0 ? 1 : 0, C1 = ax==2 ? 1 : 2). psi_q <- psi_q*b, plus
+// (difference along ax of the other field's component 3-ax-C_q) * c where
+// the slab index allows it (E side: l >= 1; H side: l <= t-2). The
+// convolution term of C_q is +psi when ax follows C_q cyclically, -psi
+// otherwise, so E[C0] gets +-(S inv)*psi_0 and E[C1] +-(S inv)*psi_1 (H with
+// the opposite sign). Everything that depends on the axis is chosen with
+// selects rather than branches, so no address is formed from values merged
+// across control flow. Returns without touching v when the cell is outside
+// the face's slab.
+template <bool IS_E, typename T>
+__device__ __forceinline__ void pml_face(const FaceDev& fd, const Dims& d, int i, int j, int k, T zx, T yx, T zy,
+                                         T xy, T yz, T xz, T k0, T k1, T k2, T& v0, T& v1, T& v2) {
+  const int ax = fd.axis;
+  const int g = ax == 0 ? i : (ax == 1 ? j : k);
+  const int n = ax == 0 ? d.nx : (ax == 1 ? d.ny : d.nz);
+  const int l = fd.high ? g - (n - fd.t) : g;
+  if (l < 0 || l >= fd.t) return;
+  // slab: the grid with the normal axis shortened to t
+  const int64_t sny = ax == 1 ? fd.t : d.ny;
+  const int64_t snz = ax == 2 ? fd.t : d.nz;
+  const int64_t li = ax == 0 ? l : i, lj = ax == 1 ? l : j, lk = ax == 2 ? l : k;
+  const int64_t off = (li * sny + lj) * snz + lk;
+  T* __restrict__ psi = reinterpret_cast<T*>(IS_E ? fd.psiE : fd.psiH);
+  const double* bt = IS_E ? fd.bE : fd.bH;
+  const double* ct = IS_E ? fd.cE : fd.cH;
+  const T b = (T)bt[l], c = (T)ct[l];
+  const T d0 = ax == 0 ? zx : (ax == 1 ? zy : yz);
+  const T d1 = ax == 0 ? yx : (ax == 1 ? xy : xz);
+  T p0 = psi[off] * b;
+  T p1 = psi[fd.slab + off] * b;
+  if (IS_E ? (l >= 1) : (l <= fd.t - 2)) {
+    p0 = p0 + d0 * c;
+    p1 = p1 + d1 * c;
+  }
+  psi[off] = p0;
+  psi[fd.slab + off] = p1;
+  const T r0 = (ax == 0 ? k1 : k0) * p0;
+  const T r1 = (ax == 2 ? k1 : k2) * p1;
+  T n0, n1, n2;
+  if (IS_E) {
+    // ax 0: E1 -= r0, E2 += r1; ax 1: E0 += r0, E2 -= r1; ax 2: E0 -= r0, E1 += r1
+    n0 = ax == 1 ? v0 + r0 : (ax == 2 ? v0 - r0 : v0);
+    n1 = ax == 0 ? v1 - r0 : (ax == 2 ? v1 + r1 : v1);
+    n2 = ax == 0 ? v2 + r1 : (ax == 1 ? v2 - r1 : v2);
+  } else {
+    n0 = ax == 1 ? v0 - r0 : (ax == 2 ? v0 + r0 : v0);
+    n1 = ax == 0 ? v1 + r0 : (ax == 2 ? v1 - r1 : v1);
+    n2 = ax == 0 ? v2 - r1 : (ax == 1 ? v2 + r1 : v2);
+  }
+  v0 = n0;
+  v1 = n1;
+  v2 = n2;
+}
+
+// All faces in `faces` that contain cell (i,j,k), in registration order.
+template <bool IS_E, typename T>
+__device__ __forceinline__ void pml_cell(const FaceDev* __restrict__ faces, int nfaces, const Dims& d, int i, int j,
+                                         int k, const Diffs<T>& df, const T kc[3], T v[3]) {
+  for (int fi = 0; fi < nfaces; ++fi)
+    pml_face<IS_E>(faces[fi], d, i, j, k, df.zx, df.yx, df.zy, df.xy, df.yz, df.xz, kc[0], kc[1], kc[2], v[0], v[1],
+                   v[2]);
+}
+
+// Object updates of one cell, registration order (objects.py:118-269); the
+// object arrays are stored as O and computed in T.
+template <typename T, typename O>
+__device__ __forceinline__ void objects_update_E(const ObjDev* __restrict__ objs, int nobjs, T sc, int i, int j,
+                                                 int k, const T curl[3], T e[3]) {
+  for (int o = 0; o < nobjs; ++o) {
+    const ObjDev& ob = objs[o];
+    if (i < ob.x0 || i >= ob.x1 || j < ob.y0 || j >= ob.y1 || k < ob.z0 || k >= ob.z1) continue;
+    const int64_t lo = ((int64_t)(i - ob.x0) * (ob.y1 - ob.y0) + (j - ob.y0)) * (ob.z1 - ob.z0) + (k - ob.z0);
+    const O* inv = reinterpret_cast<const O*>(ob.inv);
+    if (ob.kind == FDTD_OBJ_PLAIN) {
+#pragma unroll
+      for (int c = 0; c < 3; ++c) e[c] = e[c] + (sc * (T)inv[lo * 3 + c]) * curl[c];
+    } else if (ob.kind == FDTD_OBJ_ABSORBING) {
+      const O* fa = reinterpret_cast<const O*>(ob.absorb);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        const T f = (T)fa[lo * 3 + c];
+        e[c] = e[c] * ((T(1) - f) / (T(1) + f));
+        e[c] = e[c] + ((sc * (T)inv[lo * 3 + c]) * curl[c]) / (T(1) + f);
+      }
+    } else {
+      const O* m = inv + lo * 9;
+      T v[3];
+#pragma unroll
+      // the association numpy's einsum uses for this 3-term contraction
+      for (int r = 0; r < 3; ++r)
+        v[r] = ((T)m[3 * r] * curl[0] + (T)m[3 * r + 2] * curl[2]) + (T)m[3 * r + 1] * curl[1];
+#pragma unroll
+      for (int r = 0; r < 3; ++r) e[r] = e[r] + sc * v[r];
+    }
+  }
+}
+
+// Logical tile of hardware workgroup w: workgroups are dealt round-robin to
+// the 8 XCDs (each with its own L2), so consecutive logical tiles -- which
+// share halo planes -- are given to the same XCD.
+__device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned nb) {
+  const unsigned per = nb / 8, rem = nb % 8, x = w % 8, idx = w / 8;
+  return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
+// ------------------------------------------------------------ main kernels
+constexpr int FOLDED_POINTS = 4;
+
+template <class FS, class MS>
+struct MainArgs {
+  using R = typename FS::R;
+  Dims d;
+  R sc;
+  FS F;  // field updated in this half step (E or H)
+  FS G;  // field whose curl drives it (H or E)
+  MS M;  // inv_eps or inv_mu
+  const FaceDev* faces;
+  int32_t nfaces;
+  const ObjDev* objs;
+  int32_t nobjs;
+  int32_t bz_shift;  // block extent along z is 1 << bz_shift
+  int32_t cx;        // planes per workgroup along x
+  int32_t by;        // block extent along y (threads per block = bz * by)
+  // point sources folded into the E kernel (no periodic stage, no detector):
+  // their cells, in registration order, and the waveform table
+  int32_t npts;
+  int64_t pt_cell[FOLDED_POINTS];
+  const double* wave;  // row s = source s
+  int64_t wave_cap;
+  int64_t q;
+};
+
+// Main E kernel. A workgroup owns a (y, z) tile of bz*by columns (z fastest,
+// one row of bz cells per wavefront when bz = 64) and marches along x over a
+// chunk of `cx` planes: the x-neighbours come from registers of the previous
+// plane, the z-neighbours from the neighbouring lane, the y-neighbours from
+// cache.
+template <class FS, class MS, typename O, bool OBJS, bool NT>
+__global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
+  using T = typename FS::R;
+  const Dims d = a.d;
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
+  const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  const int lz = threadIdx.x & (bz - 1);
+  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
+  const bool act = k < d.nz && j < d.ny;
+  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const T zero = T(0);
+  int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
+  T hz_prev = zero, hy_prev = zero;
+  if (act && i0 > 0) {
+    hz_prev = a.G.get(2 * cs + cell - sx);
+    hy_prev = a.G.get(cs + cell - sx);
+  }
+  for (int i = i0; i < i1; ++i, cell += sx) {
+    T hx = zero, hy = zero, hz = zero;
+    if (act) {
+      hx = a.G.get(cell);
+      hy = a.G.get(cs + cell);
+      hz = a.G.get(2 * cs + cell);
+    }
+    T hy_km = __shfl_up(hy, 1);
+    T hx_km = __shfl_up(hx, 1);
+    if (act) {
+      if (lz == 0 && k > 0) {
+        hy_km = a.G.get(cs + cell - 1);
+        hx_km = a.G.get(cell - 1);
+      }
+      T hz_jm = zero, hx_jm = zero;
+      if (j > 0) {
+        hz_jm = a.G.get(2 * cs + cell - sy);
+        hx_jm = a.G.get(cell - sy);
+      }
+      // backward differences; an absent one contributes nothing (grid.py:67-74)
+      Diffs<T> df;
+      df.zy = j > 0 ? hz - hz_jm : zero;
+      df.yz = k > 0 ? hy - hy_km : zero;
+      df.xz = k > 0 ? hx - hx_km : zero;
+      df.zx = i > 0 ? hz - hz_prev : zero;
+      df.yx = i > 0 ? hy - hy_prev : zero;
+      df.xy = j > 0 ? hx - hx_jm : zero;
+      T curl[3];
+      curl[0] = (zero + df.zy) - df.yz;
+      curl[1] = (zero + df.xz) - df.zx;
+      curl[2] = (zero + df.yx) - df.xy;
+      T m[3], kc[3], e[3];
+      a.M.template get3<NT>(cell, m);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        kc[c] = a.sc * m[c];
+        e[c] = a.F.template get_s<NT>(c * cs + cell) + kc[c] * curl[c];
+      }
+      if (OBJS) objects_update_E<T, O>(a.objs, a.nobjs, a.sc, i, j, k, curl, e);
+      if (a.nfaces) pml_cell<true>(a.faces, a.nfaces, d, i, j, k, df, kc, e);
+      for (int s = 0; s < a.npts; ++s)  // sources.py:109, registration order
+        if (a.pt_cell[s] == cell) e[2] = e[2] + (T)a.wave[s * a.wave_cap + a.q];
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a.F.template put_s<NT>(c * cs + cell, e[c]);
+    }
+    hz_prev = hz;
+    hy_prev = hy;
+  }
+}
+
+// Main H kernel: the mirror image of the E kernel. Its differences are
+// forward ones, so workgroups take the tiles in decreasing order (the +y/+z
+// neighbour tiles were read just before, by the same XCD) and march along -x,
+// keeping E of plane i+1 in registers.
+template <class FS, class MS, bool NT>
+__global__ void __launch_bounds__(1024) k_main_H(MainArgs<FS, MS> a) {
+  using T = typename FS::R;
+  const Dims d = a.d;
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
+  const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
+  const unsigned tile = gridDim.x - 1 - xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  const int lz = threadIdx.x & (bz - 1);
+  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
+  const bool act = k < d.nz && j < d.ny;
+  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const T zero = T(0);
+  int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
+  T ez_next = zero, ey_next = zero;  // E at plane i+1
+  if (act && i1 < d.nx) {
+    ez_next = a.G.get(2 * cs + cell + sx);
+    ey_next = a.G.get(cs + cell + sx);
+  }
+  for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
+    T ex = zero, ey = zero, ez = zero;
+    if (act) {
+      ex = a.G.get(cell);
+      ey = a.G.get(cs + cell);
+      ez = a.G.get(2 * cs + cell);
+    }
+    T ey_kp = __shfl_down(ey, 1);
+    T ex_kp = __shfl_down(ex, 1);
+    if (act) {
+      if (lz == bz - 1 && k < d.nz - 1) {
+        ey_kp = a.G.get(cs + cell + 1);
+        ex_kp = a.G.get(cell + 1);
+      }
+      T ez_jp = zero, ex_jp = zero;
+      if (j < d.ny - 1) {
+        ez_jp = a.G.get(2 * cs + cell + sy);
+        ex_jp = a.G.get(cell + sy);
+      }
+      // forward differences; an absent one contributes nothing (grid.py:42-49)
+      Diffs<T> df;
+      df.zy = j < d.ny - 1 ? ez_jp - ez : zero;
+      df.yz = k < d.nz - 1 ? ey_kp - ey : zero;
+      df.xz = k < d.nz - 1 ? ex_kp - ex : zero;
+      df.zx = i < d.nx - 1 ? ez_next - ez : zero;
+      df.yx = i < d.nx - 1 ? ey_next - ey : zero;
+      df.xy = j < d.ny - 1 ? ex_jp - ex : zero;
+      T curl[3];
+      curl[0] = (zero + df.zy) - df.yz;
+      curl[1] = (zero + df.xz) - df.zx;
+      curl[2] = (zero + df.yx) - df.xy;
+      T m[3], kc[3], h[3];
+      a.M.template get3<NT>(cell, m);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) {
+        kc[c] = a.sc * m[c];
+        h[c] = a.F.template get_s<NT>(c * cs + cell) - kc[c] * curl[c];
+      }
+      // objects leave H unchanged (objects.py:131-137, 223-229, 271-277)
+      if (a.nfaces) pml_cell<false>(a.faces, a.nfaces, d, i, j, k, df, kc, h);
+#pragma unroll
+      for (int c = 0; c < 3; ++c) a.F.template put_s<NT>(c * cs + cell, h[c]);
+    }
+    ez_next = ez;
+    ey_next = ey;
+  }
+}
+
+// One CPML face applied on its own (faces registered after a periodic
+// boundary): F is the field being updated, G the other one. AX, the face's
+// normal axis, is a template parameter so that no index depends on a runtime
+// branch over the axis.
+template <class FS, class MS, bool IS_E, int AX>
+__global__ void __launch_bounds__(256) k_face(Dims d, typename FS::R sc, FS F, FS G, MS M, FaceDev fd) {
+  using T = typename FS::R;
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= fd.slab) return;
+  // slab-local coordinates, the slab laid out like the grid with axis AX of length t
+  const int64_t sny = AX == 1 ? fd.t : d.ny, snz = AX == 2 ? fd.t : d.nz;
+  const int64_t li = p / (sny * snz), lj = (p / snz) % sny, lk = p % snz;
+  const int l = (int)(AX == 0 ? li : (AX == 1 ? lj : lk));
+  const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
+  const int g = fd.high ? n - fd.t + l : l;
+  const int i = AX == 0 ? g : (int)li, j = AX == 1 ? g : (int)lj, k = AX == 2 ? g : (int)lk;
+  const int64_t cell = ((int64_t)i * d.ny + j) * d.nz + k;
+  const int64_t st = AX == 0 ? (int64_t)d.ny * d.nz : (AX == 1 ? (int64_t)d.nz : 1);
+  // the other field's components differenced by this face: 3-AX-C0, 3-AX-C1
+  constexpr int S0 = 3 - AX - (AX == 0 ? 1 : 0), S1 = 3 - AX - (AX == 2 ? 1 : 2);
+  const int64_t o0 = S0 * d.cs + cell, o1 = S1 * d.cs + cell;
+  T d0 = T(0), d1 = T(0);
+  if (IS_E && l >= 1) {
+    d0 = G.get(o0) - G.get(o0 - st);
+    d1 = G.get(o1) - G.get(o1 - st);
+  }
+  if (!IS_E && l <= fd.t - 2) {
+    d0 = G.get(o0 + st) - G.get(o0);
+    d1 = G.get(o1 + st) - G.get(o1);
+  }
+  T m[3], kc[3], v[3];
+  M.template get3<false>(cell, m);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    kc[c] = sc * m[c];
+    v[c] = F.get(c * d.cs + cell);
+  }
+  // pml_face reads the two normal-axis differences
+  pml_face<IS_E>(fd, d, i, j, k, d0, d1, d0, d1, d0, d1, kc[0], kc[1], kc[2], v[0], v[1], v[2]);
+#pragma unroll
+  for (int c = 0; c < 3; ++c) F.put(c * d.cs + cell, v[c]);
+}
+
+// Periodic plane copy: to_low ? plane 0 <- plane n-1 : plane n-1 <- plane 0.
+template <class FS>
+__global__ void __launch_bounds__(256) k_periodic(Dims d, FS F, int axis, int to_low) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t n_plane, src, dst;
+  if (axis == 0) {
+    n_plane = (int64_t)d.ny * d.nz;
+    if (p >= n_plane) return;
+    const int64_t last = (int64_t)(d.nx - 1) * n_plane + p;
+    src = to_low ? last : p;
+    dst = to_low ? p : last;
+  } else if (axis == 1) {
+    n_plane = (int64_t)d.nx * d.nz;
+    if (p >= n_plane) return;
+    const int64_t i = p / d.nz, k = p % d.nz;
+    const int64_t first = (i * d.ny) * d.nz + k, last = (i * d.ny + d.ny - 1) * d.nz + k;
+    src = to_low ? last : first;
+    dst = to_low ? first : last;
+  } else {
+    n_plane = (int64_t)d.nx * d.ny;
+    if (p >= n_plane) return;
+    const int64_t first = p * d.nz, last = p * d.nz + d.nz - 1;
+    src = to_low ? last : first;
+    dst = to_low ? first : last;
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) F.copy(c * d.cs + dst, c * d.cs + src);
+}
+
+template <class FS>
+struct TailArgs {
+  Dims d;
+  FS F;  // field of this half step
+  int32_t is_H;
+  int32_t nsrc, ndet;
+  const SrcDev* srcs;
+  const DetDev* dets;
+  const double* wave;
+  int64_t wave_cap;
+  int64_t q;
+  int64_t counter;  // half steps of this kind already taken by the plan
+};
+
+// Sources in registration order, then detector samples; one workgroup so the
+// ordering between sources touching the same cell and the detector reads is
+// kept by workgroup barriers.
+template <class FS>
+__global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
+  using T = typename FS::R;
+  using Sample = typename FS::Sample;
+  const int tid = threadIdx.x;
+  for (int s = 0; s < a.nsrc; ++s) {
+    const SrcDev& src = a.srcs[s];
+    if (a.is_H && src.kind != FDTD_SRC_PLANE) continue;
+    const T w = (T)a.wave[s * a.wave_cap + a.q];
+    const int64_t base = (int64_t)(a.is_H ? src.compH : src.compE) * a.d.cs;
+    for (int p = tid; p < src.n; p += blockDim.x) {
+      const int64_t cell = base + src.cells[p];
+      if (src.kind == FDTD_SRC_POINT)
+        a.F.put(cell, a.F.get(cell) + w);
+      else if (src.kind == FDTD_SRC_LINE)
+        a.F.put(cell, a.F.get(cell) + (T)src.profile[p] * w);
+      else
+        a.F.put(cell, (T)src.amplitude * w);
+    }
+    __syncthreads();
+  }
+  for (int t = 0; t < a.ndet; ++t) {
+    const DetDev& det = a.dets[t];
+    const int64_t slot = a.counter - (a.is_H ? det.slot0H : det.slot0E);
+    Sample* buf = reinterpret_cast<Sample*>(a.is_H ? det.bufH : det.bufE) + slot * det.n * 3;
+    for (int p = tid; p < det.n * 3; p += blockDim.x) {
+      const int c = p % 3;
+      buf[p] = a.F.sample(c * a.d.cs + det.cells[p / 3]);
+    }
+  }
+}
+
+// Plain device copy of n 16-byte words: the bandwidth reference for the
+// step kernels (a stream that reads and writes every byte once).
+__global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+// ================================================================ adjoint
+// The step is linear in (E, H, psi) for fixed materials, so reverse-mode
+// differentiation of an objective through T steps applies the transposes of
+// the two half steps in reverse order. The kernels below are those exact
+// transposes for plain storage with every CPML face folded into the step and
+// objects composed into a per-cell affine E update
+//   E' = dcoef * E + gcoef * curl_H(H) + CPML + sources.
+// Sources add constants and drop out of the transpose.
+
+struct Box {
+  int32_t x0, x1, y0, y1, z0, z1;  // half-open
+};
+
+__host__ __device__ __forceinline__ int64_t box_cells(const Box& b) {
+  return (int64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
+}
+
+// index of cell (i,j,k) inside box b (z fastest), or -1
+__device__ __forceinline__ int64_t box_index(const Box& b, int i, int j, int k) {
+  if (i < b.x0 || i >= b.x1 || j < b.y0 || j >= b.y1 || k < b.z0 || k >= b.z1) return -1;
+  return ((int64_t)(i - b.x0) * (b.y1 - b.y0) + (j - b.y0)) * (b.z1 - b.z0) + (k - b.z0);
+}
+
+__device__ __forceinline__ int64_t box_cell(const Box& b, const Dims& d, int64_t p) {
+  const int64_t nz = b.z1 - b.z0, ny = b.y1 - b.y0;
+  const int64_t k = b.z0 + p % nz, j = b.y0 + (p / nz) % ny, i = b.x0 + p / (nz * ny);
+  return (i * d.ny + j) * d.nz + k;
+}
+
+// After an E half step: the curl of H inside the design box (the curl that
+// half step used; H is unchanged by it) and E inside the objective box,
+// each (3, cells of the box).
+template <typename T>
+__global__ void __launch_bounds__(256) k_record(Dims d, const T* __restrict__ E, const T* __restrict__ H, Box db,
+                                                T* __restrict__ curl_out, Box pb, T* __restrict__ plane_out) {
+  const int64_t nd = box_cells(db), np = box_cells(pb);
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  if (p < nd) {
+    const int64_t cell = box_cell(db, d, p);
+    const int k = (int)(cell % d.nz), j = (int)((cell / d.nz) % d.ny), i = (int)(cell / sx);
+    const T* Hx = H;
+    const T* Hy = H + cs;
+    const T* Hz = H + 2 * cs;
+    const T zero = T(0);
+    const T zy = j > 0 ? Hz[cell] - Hz[cell - sy] : zero, yz = k > 0 ? Hy[cell] - Hy[cell - 1] : zero;
+    const T xz = k > 0 ? Hx[cell] - Hx[cell - 1] : zero, zx = i > 0 ? Hz[cell] - Hz[cell - sx] : zero;
+    const T yx = i > 0 ? Hy[cell] - Hy[cell - sx] : zero, xy = j > 0 ? Hx[cell] - Hx[cell - sy] : zero;
+    curl_out[p] = (zero + zy) - yz;
+    curl_out[nd + p] = (zero + xz) - zx;
+    curl_out[2 * nd + p] = (zero + yx) - xy;
+  } else if (p < nd + np) {
+    const int64_t q = p - nd;
+    const int64_t cell = box_cell(pb, d, q);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) plane_out[c * np + q] = E[c * cs + cell];
+  }
+}
+
+// Adjoint CPML state of one face: psi of the E and of the H side, two
+// buffers each, read and written alternately (input `parity`, output
+// 1 - parity), (2, slab) values per buffer like the forward psi.
+struct AdjFaceDev {
+  void* psiE[2];
+  void* psiH[2];
+};
+
+template <typename T>
+struct AdjArgs {
+  Dims d;
+  T sc;
+  T* lamE;               // dJ/dE, (3, N)
+  T* lamH;               // dJ/dH, (3, N)
+  const T* inv_eps;      // grid materials: CPML corrections use S*inv
+  const T* inv_mu;
+  const T* gcoef;        // (3, N)
+  const T* dcoef;        // (3, N) or null (all ones)
+  const FaceDev* faces;  // CPML faces in registration order (b, c profiles)
+  const AdjFaceDev* afaces;
+  int32_t nfaces;
+  int32_t parity;
+  Box db;                // design box: grad += S lamE curl
+  const T* curl;         // (3, cells of db) recorded for this step
+  double* grad;          // (3, cells of db)
+  Box pb;                // objective box: lamE += 2 E
+  const T* plane;        // (3, cells of pb) recorded for this step
+};
+
+// Slab index of cell (i,j,k) in face fd, or -1.
+__device__ __forceinline__ int face_l(const FaceDev& fd, const Dims& d, int i, int j, int k) {
+  const int ax = fd.axis;
+  const int g = ax == 0 ? i : (ax == 1 ? j : k);
+  const int n = ax == 0 ? d.nx : (ax == 1 ? d.ny : d.nz);
+  const int l = fd.high ? g - (n - fd.t) : g;
+  return (l >= 0 && l < fd.t) ? l : -1;
+}
+
+// Offset of cell (i,j,k) with slab index l in the face's (slab) layout.
+__device__ __forceinline__ int64_t face_off(const FaceDev& fd, const Dims& d, int i, int j, int k, int l) {
+  const int ax = fd.axis;
+  const int64_t sny = ax == 1 ? fd.t : d.ny, snz = ax == 2 ? fd.t : d.nz;
+  const int64_t li = ax == 0 ? l : i, lj = ax == 1 ? l : j, lk = ax == 2 ? l : k;
+  return (li * sny + lj) * snz + lk;
+}
+
+// Transpose of the H half step (grid.py:301-325, boundaries.py:421-431 and
+// 461-487). Forward, per face and psi component q (E component s_q = 3-ax-C_q
+// differenced along the normal):
+//   psiH'_q = bH psiH_q + cH(l) (E_s(l+1) - E_s(l))     for l <= t-2
+//   H'      = H - K_H curl_E(E) + sg_q K_H psiH'_q,       K_H = S inv_mu
+// so with w_q = lamPsiH'_q + sg_q K_H lamH (lamH = dJ/dH' = dJ/dH):
+//   lamPsiH_q = bH w_q
+//   lamE     += -curl_E^T(K_H lamH) + transpose of the forward difference.
+// curl_E^T gathers backward neighbours with the masks of curl_E
+// (grid.py:39-49): (curl_E^T Y)_x = ([k>=1] Y_y(k-1) - [k<=nz-2] Y_y(k))
+// - ([j>=1] Y_z(j-1) - [j<=ny-2] Y_z(j)), and cyclically.
+// Two fused extras: lamE is first scaled by dcoef (deferred from the
+// transpose of the following E half step) and, after the update, the
+// objective's 2 E is added inside the objective box.
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_H(AdjArgs<T> a) {
+  const Dims d = a.d;
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= d.cs) return;
+  const int64_t cs = d.cs, sy = d.nz, sx = (int64_t)d.ny * d.nz;
+  const int k = (int)(cell % d.nz), j = (int)((cell / d.nz) % d.ny), i = (int)(cell / sx);
+  const T sc = a.sc;
+  auto Y = [&](int c, int64_t x) { return (sc * a.inv_mu[c * cs + x]) * a.lamH[c * cs + x]; };
+  const T zero = T(0);
+  const T y0 = Y(0, cell), y1 = Y(1, cell), y2 = Y(2, cell);
+  const T t_x = ((k >= 1 ? Y(1, cell - 1) : zero) - (k <= d.nz - 2 ? y1 : zero)) -
+                ((j >= 1 ? Y(2, cell - sy) : zero) - (j <= d.ny - 2 ? y2 : zero));
+  const T t_y = -((k >= 1 ? Y(0, cell - 1) : zero) - (k <= d.nz - 2 ? y0 : zero)) +
+                ((i >= 1 ? Y(2, cell - sx) : zero) - (i <= d.nx - 2 ? y2 : zero));
+  const T t_z = ((j >= 1 ? Y(0, cell - sy) : zero) - (j <= d.ny - 2 ? y0 : zero)) -
+                ((i >= 1 ? Y(1, cell - sx) : zero) - (i <= d.nx - 2 ? y1 : zero));
+  T le[3];
+#pragma unroll
+  for (int c = 0; c < 3; ++c) le[c] = a.dcoef ? a.dcoef[c * cs + cell] * a.lamE[c * cs + cell] : a.lamE[c * cs + cell];
+  le[0] -= t_x;
+  le[1] -= t_y;
+  le[2] -= t_z;
+  for (int f = 0; f < a.nfaces; ++f) {
+    const FaceDev& fd = a.faces[f];
+    const int ax = fd.axis;
+    const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
+    const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
+    // phi_H = psi_{C+1} - psi_{C+2} (indices mod 3) enters H' with a minus
+    // sign: psi_q along ax has weight -K_H when ax = C_q + 1, +K_H otherwise
+    const T sg0 = ax == (c0 + 1) % 3 ? T(-1) : T(1), sg1 = ax == (c1 + 1) % 3 ? T(-1) : T(1);
+    const T* pin = reinterpret_cast<const T*>(a.afaces[f].psiH[a.parity]);
+    T* pout = reinterpret_cast<T*>(a.afaces[f].psiH[1 - a.parity]);
+    const int l = face_l(fd, d, i, j, k);
+    if (l >= 0) {
+      const int64_t off = face_off(fd, d, i, j, k, l);
+      const T w0 = pin[off] + sg0 * Y(c0, cell);
+      const T w1 = pin[fd.slab + off] + sg1 * Y(c1, cell);
+      const T b = (T)fd.bH[l];
+      pout[off] = b * w0;
+      pout[fd.slab + off] = b * w1;
+      if (l <= fd.t - 2) {  // psi'(l) holds c(l) (E(l+1) - E(l))
+        const T c = (T)fd.cH[l];
+        le[s0] -= c * w0;
+        le[s1] -= c * w1;
+      }
+    }
+    // ... and E at this cell is the E(l+1) of the face cell one step back
+    const int64_t st = ax == 0 ? sx : (ax == 1 ? sy : 1);
+    const int ib = ax == 0 ? i - 1 : i, jb = ax == 1 ? j - 1 : j, kb = ax == 2 ? k - 1 : k;
+    if (ib >= 0 && jb >= 0 && kb >= 0) {
+      const int lb = face_l(fd, d, ib, jb, kb);
+      if (lb >= 0 && lb <= fd.t - 2) {
+        const int64_t nb = cell - st;
+        const int64_t off = face_off(fd, d, ib, jb, kb, lb);
+        const T w0 = pin[off] + sg0 * Y(c0, nb);
+        const T w1 = pin[fd.slab + off] + sg1 * Y(c1, nb);
+        const T c = (T)fd.cH[lb];
+        le[s0] += c * w0;
+        le[s1] += c * w1;
+      }
+    }
+  }
+  const int64_t pp = box_index(a.pb, i, j, k);
+  if (pp >= 0) {
+    const int64_t np = box_cells(a.pb);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) le[c] += T(2) * a.plane[c * np + pp];
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) a.lamE[c * cs + cell] = le[c];
+}
+
+// Transpose of the E half step (grid.py:275-299, objects.py:118-221,
+// boundaries.py:409-459). Forward, per face and psi component q (H component
+// s_q = 3-ax-C_q differenced along the normal):
+//   psiE'_q = bE psiE_q + cE(l) (H_s(l) - H_s(l-1))      for l >= 1
+//   E'      = dcoef E + gcoef curl_H(H) + sg_q K_E psiE'_q, K_E = S inv_eps
+// so with w_q = lamPsiE'_q + sg_q K_E lamE (lamE = dJ/dE'):
+//   lamPsiE_q = bE w_q
+//   lamH     += curl_H^T(gcoef lamE) + transpose of the backward difference
+//   lamE      = dcoef lamE   (left to the next k_adj_H, which reads lamE
+//                             only at its own cell)
+// curl_H^T gathers forward neighbours with the masks of curl_H
+// (grid.py:65-73): (curl_H^T Y)_x = ([k>=1] Y_y - [k<=nz-2] Y_y(k+1))
+// - ([j>=1] Y_z - [j<=ny-2] Y_z(j+1)), and cyclically. Fused: the design
+// gradient grad += S lamE curl (dE'/d inv = S curl inside the design box).
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_E(AdjArgs<T> a) {
+  const Dims d = a.d;
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= d.cs) return;
+  const int64_t cs = d.cs, sy = d.nz, sx = (int64_t)d.ny * d.nz;
+  const int k = (int)(cell % d.nz), j = (int)((cell / d.nz) % d.ny), i = (int)(cell / sx);
+  const T sc = a.sc;
+  auto Y = [&](int c, int64_t x) { return a.gcoef[c * cs + x] * a.lamE[c * cs + x]; };
+  auto KL = [&](int c, int64_t x) { return (sc * a.inv_eps[c * cs + x]) * a.lamE[c * cs + x]; };
+  const T zero = T(0);
+  const T y0 = Y(0, cell), y1 = Y(1, cell), y2 = Y(2, cell);
+  const T t_x = ((k >= 1 ? y1 : zero) - (k <= d.nz - 2 ? Y(1, cell + 1) : zero)) -
+                ((j >= 1 ? y2 : zero) - (j <= d.ny - 2 ? Y(2, cell + sy) : zero));
+  const T t_y = -((k >= 1 ? y0 : zero) - (k <= d.nz - 2 ? Y(0, cell + 1) : zero)) +
+                ((i >= 1 ? y2 : zero) - (i <= d.nx - 2 ? Y(2, cell + sx) : zero));
+  const T t_z = ((j >= 1 ? y0 : zero) - (j <= d.ny - 2 ? Y(0, cell + sy) : zero)) -
+                ((i >= 1 ? y1 : zero) - (i <= d.nx - 2 ? Y(1, cell + sx) : zero));
+  T lh[3] = {a.lamH[cell] + t_x, a.lamH[cs + cell] + t_y, a.lamH[2 * cs + cell] + t_z};
+  for (int f = 0; f < a.nfaces; ++f) {
+    const FaceDev& fd = a.faces[f];
+    const int ax = fd.axis;
+    const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
+    const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
+    // phi_E = psi_{C+1} - psi_{C+2} (indices mod 3) enters E' with a plus
+    // sign: psi_q along ax has weight +K_E when ax = C_q + 1, -K_E otherwise
+    const T sg0 = ax == (c0 + 1) % 3 ? T(1) : T(-1), sg1 = ax == (c1 + 1) % 3 ? T(1) : T(-1);
+    const T* pin = reinterpret_cast<const T*>(a.afaces[f].psiE[a.parity]);
+    T* pout = reinterpret_cast<T*>(a.afaces[f].psiE[1 - a.parity]);
+    const int l = face_l(fd, d, i, j, k);
+    if (l >= 0) {
+      const int64_t off = face_off(fd, d, i, j, k, l);
+      const T w0 = pin[off] + sg0 * KL(c0, cell);
+      const T w1 = pin[fd.slab + off] + sg1 * KL(c1, cell);
+      const T b = (T)fd.bE[l];
+      pout[off] = b * w0;
+      pout[fd.slab + off] = b * w1;
+      if (l >= 1) {  // psi'(l) holds c(l) (H(l) - H(l-1))
+        const T c = (T)fd.cE[l];
+        lh[s0] += c * w0;
+        lh[s1] += c * w1;
+      }
+    }
+    // ... and H at this cell is the H(l-1) of the face cell one step ahead
+    const int64_t st = ax == 0 ? sx : (ax == 1 ? sy : 1);
+    const int iu = ax == 0 ? i + 1 : i, ju = ax == 1 ? j + 1 : j, ku = ax == 2 ? k + 1 : k;
+    if (iu < d.nx && ju < d.ny && ku < d.nz) {
+      const int lu = face_l(fd, d, iu, ju, ku);
+      if (lu >= 1) {
+        const int64_t nu = cell + st;
+        const int64_t off = face_off(fd, d, iu, ju, ku, lu);
+        const T w0 = pin[off] + sg0 * KL(c0, nu);
+        const T w1 = pin[fd.slab + off] + sg1 * KL(c1, nu);
+        const T c = (T)fd.cE[lu];
+        lh[s0] -= c * w0;
+        lh[s1] -= c * w1;
+      }
+    }
+  }
+#pragma unroll
+  for (int c = 0; c < 3; ++c) a.lamH[c * cs + cell] = lh[c];
+  const int64_t pd = box_index(a.db, i, j, k);
+  if (pd >= 0) {
+    const int64_t nd = box_cells(a.db);
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+      a.grad[c * nd + pd] += (double)sc * (double)a.lamE[c * cs + cell] * (double)a.curl[c * nd + pd];
+  }
+}
+
+}  // namespace fdtd_dev

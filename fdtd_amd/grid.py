@@ -80,15 +80,28 @@ class Grid:
             self.courant_number = float(courant_number)
         self.time_step = self.courant_number * self.grid_spacing / const.c
 
-        self.dtype = bd.float
+        self.dtype = bd.float                  # fields
+        self.mat_dtype = bd.material_float     # materials, object arrays, CPML state
+        self.compensated = bool(bd.compensated)
         dev = bd.device
         soa = (3, self.Nx, self.Ny, self.Nz)
         self._E = torch.zeros(soa, dtype=self.dtype, device=dev)
         self._H = torch.zeros(soa, dtype=self.dtype, device=dev)
-        self._inv_eps = torch.empty(soa, dtype=self.dtype, device=dev)
-        self._inv_mu = torch.empty(soa, dtype=self.dtype, device=dev)
+        self._inv_eps = torch.empty(soa, dtype=self.mat_dtype, device=dev)
+        self._inv_mu = torch.empty(soa, dtype=self.mat_dtype, device=dev)
         self._fill_inverse(self._inv_eps, permittivity)
         self._fill_inverse(self._inv_mu, permeability)
+        if self.compensated:
+            # 8-bit compensation terms of the float32 fields (reset whenever the
+            # host writes the fields, see _sync_compensation)
+            self._E_lo = torch.zeros(soa, dtype=torch.int8, device=dev)
+            self._H_lo = torch.zeros(soa, dtype=torch.int8, device=dev)
+            self._field_versions = (self._E._version, self._H._version)
+        # materials read through a <=256-entry float64 palette: "auto" uses one
+        # in the compensated mode when the grid has few distinct materials
+        self.material_palette = "auto"
+        self._palette = None
+        self._palette_key = None
 
         self.time_steps_passed = 0
         self.sources = []
@@ -232,8 +245,46 @@ class Grid:
     def _components(self):
         return list(self.boundaries) + list(self.objects) + list(self.sources) + list(self.detectors)
 
+    def _palette_wanted(self):
+        mode = self.material_palette
+        return bool(self.compensated) if mode == "auto" else bool(mode)
+
+    def _material_palette(self):
+        """(eps_id, eps_table, mu_id, mu_table) or None when a grid material has
+        more than 256 distinct (x, y, z) triples. Rebuilt when the material
+        arrays change (their torch version counters)."""
+        key = (self._inv_eps._version, self._inv_mu._version, self._inv_eps.data_ptr(), self._inv_mu.data_ptr())
+        if key == self._palette_key:
+            return self._palette
+        out = []
+        for arr in (self._inv_eps, self._inv_mu):
+            rows = arr.reshape(3, -1).transpose(0, 1)
+            uniq, idx = torch.unique(rows, dim=0, return_inverse=True)
+            if uniq.shape[0] > 256:
+                out = None
+                break
+            tab = torch.zeros((256, 3), dtype=torch.float64, device=arr.device)
+            tab[: uniq.shape[0]] = uniq.to(torch.float64)
+            out += [idx.to(torch.uint8).contiguous(), tab]
+        self._palette = tuple(out) if out else None
+        self._palette_key = key
+        return self._palette
+
+    def _sync_compensation(self):
+        """Host writes to the float32 fields invalidate their compensation terms."""
+        if not self.compensated:
+            return
+        if self._E._version != self._field_versions[0]:
+            self._E_lo.zero_()
+        if self._H._version != self._field_versions[1]:
+            self._H_lo.zero_()
+        self._field_versions = (self._E._version, self._H._version)
+
     def _current_plan_key(self):
-        key = [len(self.boundaries), len(self.objects), len(self.sources), len(self.detectors), self.courant_number]
+        key = [len(self.boundaries), len(self.objects), len(self.sources), len(self.detectors), self.courant_number,
+               self._palette_wanted()]
+        if self._palette_wanted():
+            key += [self._inv_eps._version, self._inv_mu._version]
         for comp in self._components():
             key.append(id(comp))
             key.extend(comp._plan_key())
@@ -245,9 +296,17 @@ class Grid:
         key = self._current_plan_key()
         if self._plan is not None and key == self._plan_key:
             return self._plan
-        dtype_code = _native.F64 if self.dtype == torch.float64 else _native.F32
+        if self.compensated:
+            dtype_code = _native.F32C
+        else:
+            dtype_code = _native.F64 if self.dtype == torch.float64 else _native.F32
         plan = _native.Plan(dtype_code, (self.Nx, self.Ny, self.Nz), self.courant_number,
                             self._E, self._H, self._inv_eps, self._inv_mu)
+        if self.compensated:
+            plan.set_compensation(self._E_lo, self._H_lo)
+        pal = self._material_palette() if self._palette_wanted() else None
+        if pal is not None:
+            plan.set_material_palette(*pal)
         for comp in self._components():
             comp._add_to_plan(plan)
         self._plan = plan
@@ -276,6 +335,7 @@ class Grid:
         self._plan.set_waveforms(self._wave)
 
     def _prepare(self, nsteps, e_steps, h_steps):
+        self._sync_compensation()
         plan = self._ensure_plan()
         grown = False
         for det in self.detectors:

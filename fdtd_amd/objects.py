@@ -100,10 +100,10 @@ class Object:
         """Bind `attr` as a contiguous device tensor of the field dtype."""
         v = getattr(self, attr)
         g = self.grid
-        ok = (torch.is_tensor(v) and v.device == g._E.device and v.dtype == g.dtype and v.is_contiguous()
+        ok = (torch.is_tensor(v) and v.device == g._E.device and v.dtype == g.mat_dtype and v.is_contiguous()
               and tuple(v.shape) == shape)
         if not ok:
-            t = torch.as_tensor(_host(v) if not torch.is_tensor(v) else v).to(device=g._E.device, dtype=g.dtype)
+            t = torch.as_tensor(_host(v) if not torch.is_tensor(v) else v).to(device=g._E.device, dtype=g.mat_dtype)
             t = t.expand(shape).contiguous()
             setattr(self, attr, t)
             v = t

@@ -16,7 +16,10 @@
  *   - Fields E, H and the grid materials are stored component-major:
  *     (3, Nx, Ny, Nz), z fastest. Object arrays keep the reference layout
  *     (nx, ny, nz, 3) or (nx, ny, nz, 3, 3).
- *   - dtype: FDTD_F64 or FDTD_F32 for every floating array of the plan.
+ *   - dtype: FDTD_F64 or FDTD_F32 for every floating array of the plan, or
+ *     FDTD_F32C: E and H stored as float32 plus an 8-bit compensation term per
+ *     value (fdtd_plan_set_compensation), arithmetic in float64, materials,
+ *     object arrays and CPML state in float64, detector traces in float32.
  *   - Every function returns 0 on success and a negative code on error;
  *     fdtd_last_error() then describes the failure. No C++ exception crosses
  *     the ABI.
@@ -33,7 +36,7 @@ extern "C" {
 
 typedef struct FdtdPlan FdtdPlan;
 
-enum { FDTD_F64 = 0, FDTD_F32 = 1 };
+enum { FDTD_F64 = 0, FDTD_F32 = 1, FDTD_F32C = 2 };
 enum { FDTD_OBJ_PLAIN = 0, FDTD_OBJ_ABSORBING = 1, FDTD_OBJ_ANISOTROPIC = 2 };
 enum { FDTD_SRC_POINT = 0, FDTD_SRC_LINE = 1, FDTD_SRC_PLANE = 2 };
 
@@ -52,6 +55,19 @@ int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number,
                      FdtdPlan** out);
 
 int fdtd_plan_destroy(FdtdPlan* plan);
+
+/* FDTD_F32C only, required: the 8-bit compensation terms of E and H, int8
+ * (3, nx, ny, nz) each. The stored value is E + q * 2^(e-32), e the binary
+ * exponent of the float E in frexp's convention. */
+int fdtd_plan_set_compensation(FdtdPlan* plan, void* E_lo, void* H_lo);
+
+/* Optional, any dtype: read the grid materials through a palette instead of
+ * the (3, nx, ny, nz) arrays. eps_id / mu_id: uint8 (nx, ny, nz) indices into
+ * float64 (256, 3) tables of inverse permittivity / permeability triples (all
+ * four device pointers, or all NULL to read the arrays again). The caller
+ * rebuilds the palette when the grid materials change. */
+int fdtd_plan_set_material_palette(FdtdPlan* plan, const uint8_t* eps_id, const double* eps_table,
+                                   const uint8_t* mu_id, const double* mu_table);
 
 /* One CPML face (fdtd/boundaries.py:225-487 with the face subclasses
  * :490-625). axis 0/1/2, high 0 (low face) or 1 (high face), thickness t.
@@ -113,6 +129,59 @@ int fdtd_half_step_H(FdtdPlan* plan, int64_t q, void* stream);
 /* nsteps full steps starting at time index q0 (fdtd/grid.py:250-273):
  * E half step then H half step at q, then q+1. */
 int fdtd_run(FdtdPlan* plan, int64_t q0, int64_t nsteps, void* stream);
+
+/* ---- Differentiation (reverse mode) -------------------------------------
+ * The reference differentiates through its torch backend (fdtd/backend.py:
+ * 207-361 with autograd enabled): torch records every array operation of
+ * Grid.update_E/update_H and replays their transposes. These entry points
+ * replace that for one objective, J = sum over the steps of sum over the
+ * cells of an objective box of |E|^2 (E after each step), and its gradient
+ * with respect to the inverse permittivity of a design box:
+ *   fdtd_run_recorded  forward steps (as fdtd_run) that also record, after
+ *                      each E half step, curl H inside the design box and E
+ *                      inside the objective box;
+ *   fdtd_adjoint_run   the transposed steps in reverse order over such a
+ *                      record, accumulating dJ/d(inverse permittivity).
+ * Plans with periodic boundaries, anisotropic objects or FDTD_F32C storage
+ * are rejected. */
+typedef struct FdtdAdjoint FdtdAdjoint;
+
+/* Elements of the adjoint CPML state buffer (plan dtype) that
+ * fdtd_adjoint_create needs for this plan: 8 * (cells of each face slab). */
+int fdtd_adjoint_psi_size(const FdtdPlan* plan, int64_t* elements);
+
+/* Adjoint state over a finalized plan. Device arrays, plan dtype, borrowed:
+ *   gcoef, dcoef  (3, Nx, Ny, Nz): the E update as E' = dcoef E + gcoef
+ *                 curl_H(H) + CPML + sources, composed by the caller from the
+ *                 grid and the objects (S inv_eps outside objects, S inv_obj
+ *                 inside a plain object, absorbing objects scale both);
+ *                 dcoef may be NULL (all ones)
+ *   lamE, lamH    (3, Nx, Ny, Nz) adjoint fields, zeroed by the caller
+ *   psi_adj       fdtd_adjoint_psi_size elements, zeroed by the caller
+ *   grad          float64 (3, cells of the design box), z fastest, zeroed by
+ *                 the caller; fdtd_adjoint_run adds to it
+ * Boxes are half-open {x0, x1, y0, y1, z0, z1} inside the grid. */
+int fdtd_adjoint_create(FdtdPlan* plan, const void* gcoef, const void* dcoef, void* lamE, void* lamH, void* psi_adj,
+                        const int32_t design_box[6], const int32_t objective_box[6], double* grad,
+                        FdtdAdjoint** out);
+
+int fdtd_adjoint_destroy(FdtdAdjoint* adj);
+
+/* nsteps forward steps from time index q0 through the adjoint's plan. After
+ * the E half step of local step s: curl_records[s] = curl H inside the design
+ * box (3, cells) and plane_records[s] = E inside the objective box (3, cells),
+ * plan dtype. */
+int fdtd_run_recorded(FdtdAdjoint* adj, int64_t q0, int64_t nsteps, void* curl_records, void* plane_records,
+                      void* stream);
+
+/* The transposed steps for local steps nsteps-1 down to 0 of a record written
+ * by fdtd_run_recorded: with lamE/lamH/psi_adj holding dJ/d(state after the
+ * last of these steps), leave dJ/d(state before the first) and add the
+ * segment's share to grad. Segments are processed latest first. lamE is kept
+ * one factor behind: the dcoef of the last transposed E update is applied by
+ * the next call, so with absorbing objects the final lamE lacks it. */
+int fdtd_adjoint_run(FdtdAdjoint* adj, int64_t nsteps, const void* curl_records, const void* plane_records,
+                     void* stream);
 
 /* Measurement hook, no counterpart in the reference (used by bench.py to
  * price the dominant kernels against the HBM roofline): launch the fused main

@@ -1,0 +1,136 @@
+"""The gradient fixture and the gradient API's host logic (no GPU).
+
+adjoint_g5.npz holds dJ/d permittivity of the 48x48x24 gradient scenario as
+the reference's torch backend computes it with autograd. Here it is pinned
+independently: its J and final E against the oracle's forward run, and its
+gradient against central differences of the oracle's J.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import adjoint_scenarios as A
+import fdtd_amd as fdtd
+from fdtd_amd.adjoint import IntensityGradient, _box_of
+from fdtd_amd.backend import host_only
+from oracle import fdtd_oracle as orc
+
+
+def rel(a, b):
+    a, b = np.asarray(a, np.float64), np.asarray(b, np.float64)
+    return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
+
+
+@pytest.fixture(scope="module")
+def g5(golden):
+    return golden("adjoint_g5")
+
+
+@pytest.fixture(scope="module")
+def golden():
+    import os
+
+    from conftest import GOLDEN
+
+    return lambda name: np.load(os.path.join(GOLDEN, f"{name}.npz"))
+
+
+def test_fixture_inputs_regenerate(g5):
+    assert np.array_equal(g5["permittivity"], A.design_permittivity(A.G5))
+
+
+def test_oracle_forward_matches_fixture(g5):
+    sc = A.G5
+    sim = A.build_oracle(orc, sc, A.design_permittivity(sc))
+    J = 0.0
+    for _ in range(sc["steps"]):
+        sim.step()
+        J += float((sim.E[sc["plane"]] ** 2).sum())
+    assert abs(J - float(g5["J"])) <= 1e-12 * float(g5["J"])
+    assert rel(sim.E, g5["E"]) <= 1e-12
+
+
+@pytest.mark.slow
+def test_fixture_gradient_matches_oracle_central_differences(g5):
+    """Two voxels (the one of largest |grad| and a seeded random one):
+    (J(eps+h) - J(eps-h)) / 2h of the oracle against the fixture."""
+    sc = A.G5
+    perm = A.design_permittivity(sc)
+    grad = g5["grad"]
+    vox = [np.unravel_index(int(np.argmax(np.abs(grad))), grad.shape), A.sample_voxels(sc, 1, seed=3)[0]]
+    for v in vox:
+        v = tuple(int(i) for i in v)
+        h = 1e-4 * perm[v]
+        pp, pm = perm.copy(), perm.copy()
+        pp[v] += h
+        pm[v] -= h
+        fd = (A.oracle_objective(orc, sc, pp) - A.oracle_objective(orc, sc, pm)) / (2 * h)
+        assert abs(fd - grad[v]) <= 1e-6 * abs(fd), (v, fd, grad[v])
+
+
+@pytest.fixture
+def host():
+    host_only(True)
+    fdtd.set_backend("hip.float64")
+    yield
+    host_only(False)
+
+
+def test_gradient_api_validation(host):
+    if True:
+        fdtd.set_backend("hip.float64")
+        sc = A.TINY
+        grid, design = A.build(fdtd, sc, torch.from_numpy(A.design_permittivity(sc)))
+        prob = IntensityGradient(grid, design, sc["plane"], sc["steps"])
+        assert prob.box == (15, 16, 6, 12, 5, 9)
+        assert prob.nd == 5 * 6 * 6 and prob.np == 6 * 4
+        assert _box_of(grid, (slice(-3, None), 2, slice(None))) == (17, 20, 2, 3, 0, 14)
+        with pytest.raises(IndexError):
+            _box_of(grid, (slice(0, 4, 2), 1, 1))
+        with pytest.raises(IndexError):
+            _box_of(grid, (1, 1))
+        with pytest.raises(ValueError):
+            IntensityGradient(grid, grid.pml_xlow, sc["plane"], 10)
+        with pytest.raises(ValueError):
+            IntensityGradient(grid, design, sc["plane"], 10, checkpoint="every")
+        with pytest.raises(ValueError):
+            IntensityGradient(grid, design, sc["plane"], 0)
+        # an object overlapping the design region
+        grid[9:11, 7:9, 5:7] = fdtd.Object(2.0, name="overlap")
+        with pytest.raises(ValueError):
+            IntensityGradient(grid, design, sc["plane"], 10)
+        # periodic boundaries and anisotropic objects have no transposed kernels
+        g2, d2 = A.build(fdtd, sc, torch.from_numpy(A.design_permittivity(sc)))
+        g2[0:2, 0:2, 0:2] = fdtd.AnisotropicObject(2.0, name="aniso")
+        with pytest.raises(NotImplementedError):
+            IntensityGradient(g2, d2, sc["plane"], 10)
+        fdtd.set_backend("hip.float32c")
+        g3, d3 = A.build(fdtd, sc, torch.from_numpy(A.design_permittivity(sc)))
+        with pytest.raises(NotImplementedError):
+            IntensityGradient(g3, d3, sc["plane"], 10)
+        fdtd.set_backend("hip.float64")
+
+
+def test_design_permittivity_border_and_chain_rule(host):
+    """The design's border entries keep the grid's values (the reference's
+    registration, objects.py:79-90); the rest is 1/permittivity."""
+    if True:
+        fdtd.set_backend("hip.float64")
+        sc = A.TINY
+        perm = A.design_permittivity(sc)
+        grid, design = A.build(fdtd, sc, torch.from_numpy(perm))
+        prob = IntensityGradient(grid, design, sc["plane"], sc["steps"])
+        before = design.inverse_permittivity.clone()
+        newp = perm * 1.5
+        prob._set_permittivity(torch.from_numpy(newp))
+        inv = design.inverse_permittivity
+        border = prob._border
+        assert torch.equal(inv[border], before[border])
+        expect = (1.0 / torch.from_numpy(newp))[..., None].expand(inv.shape)
+        assert torch.allclose(inv[~border], expect[~border], rtol=0, atol=0)
+        g, dcoef = prob._coefficients()
+        assert dcoef is None
+        S = grid.courant_number
+        x, y, z = design.x, design.y, design.z
+        assert torch.allclose(g[:, x, y, z], S * inv.permute(3, 0, 1, 2), rtol=1e-15, atol=0)

@@ -239,3 +239,43 @@ def test_float64_folded_point_sources_match_oracle(seed):
     bitwise = not any(k[0] == "AnisotropicObject" for k in sc["components"])
     _assert_same(_np(grid.E), sim.E, "E", bitwise)
     _assert_same(_np(grid.H), sim.H, "H", bitwise)
+
+
+@pytest.mark.parametrize("name", ["c1", "manual", "oned", "float_keys", "periodic_z_last"])
+def test_material_palette_is_bit_invisible_in_float64(golden, name):
+    """Reading the grid materials through the palette gives the same bits as
+    reading the per-cell arrays (the palette stores the same float64 values);
+    grids with more than 256 distinct materials fall back to the arrays."""
+    g = golden(name)
+    sc = scenarios.get_scenario(name)
+    grid, _ = _build(sc)
+    grid.material_palette = True
+    scenarios.run_program(grid, sc)
+    torch.cuda.synchronize()
+    _assert_same(_np(grid.E), g["E"], "E")
+    _assert_same(_np(grid.H), g["H"], "H")
+
+
+@pytest.mark.parametrize("name", ["g2", "c1", "manual", "periodic_x_first", "periodic_y_last", "oned"])
+def test_compensated_float32_error_against_float64_reference(golden, name):
+    """hip.float32c (float32 storage + 8-bit compensation, float64 arithmetic):
+    relative L2 error against the float64 reference stated and bounded by
+    1e-7 -- two to three orders below plain float32. "manual" writes a hard
+    source into E from the host every step; the host sees float32 tensors,
+    so each written value is rounded to float32 (2^-24 relative) and the
+    bound there is 1e-6."""
+    fdtd.set_backend("hip.float32c")
+    g = golden(name)
+    sc = scenarios.get_scenario(name)
+    grid, comps = scenarios.build_fdtd(fdtd, sc, to_array=torch.from_numpy)
+    scenarios.run_program(grid, sc)
+    torch.cuda.synchronize()
+    eE = rel_l2(_np(grid.E), g["E"])
+    eH = rel_l2(_np(grid.H), g["H"])
+    _say(f"[gpu] float32c {name}: relative L2 error E {eE:.2e} H {eH:.2e}")
+    bound = 1e-6 if name == "manual" else 1e-7
+    assert eE < bound and eH < bound
+    for i, c in enumerate(comps):
+        cls = str(g[f"reg/{i}/class"])
+        if cls in ("LineDetector", "BlockDetector"):
+            assert rel_l2(_np(c.trace("E")), g[f"out/{i}/E"]) < 1e-6
