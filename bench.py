@@ -53,19 +53,28 @@ PML_KEYS = lambda t: [(slice(0, t), slice(None), slice(None)), (slice(-t, None),
                       (slice(None), slice(None), slice(0, t)), (slice(None), slice(None), slice(-t, None))]
 
 
+PALETTE = "auto"  # --palette: how the grid material is read (auto | on | off)
+
+
 def build_grid(fdtd, n=256, pml=16):
     eps = sphere_permittivity(n)
     grid = fdtd.Grid((n, n, n), grid_spacing=50e-9, permittivity=eps, courant_number=3 ** -0.5)
+    grid.material_palette = {"auto": "auto", "on": True, "off": False}[PALETTE]
     for k in PML_KEYS(pml):
         grid[k] = fdtd.PML()
     grid[n // 2, n // 2, n // 2] = fdtd.PointSource(period=1550e-9 / 299792458.0, pulse=True)
     return grid
 
 
-def algorithmic_bytes_per_half_step(n, pml, itemsize):
+def algorithmic_bytes_per_half_step(n, pml, itemsize, compensated=False):
     """12 values per cell (read field, other field, material; write field) plus
-    4 per (face, slab cell) (read and write the two psi components)."""
+    4 per (face, slab cell) (read and write the two psi components). The
+    compensated form stores a field value in 5 bytes (float + int8), reads the
+    material as a 1-byte palette index and keeps psi in float64."""
     cells = n ** 3
+    if compensated:
+        slab = pml * n * n
+        return cells * (9 * 5 + 1) + 6 * slab * 4 * 8
     slab = pml * n * n
     return cells * 12 * itemsize + 6 * slab * 4 * itemsize
 
@@ -135,7 +144,7 @@ def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps):
     ms = ev0.elapsed_time(ev1)
     ms_step = ms / steps
     itemsize = 8 if dtype == "float64" else 4
-    half_bytes = algorithmic_bytes_per_half_step(n, pml, itemsize)
+    half_bytes = algorithmic_bytes_per_half_step(n, pml, itemsize, compensated=dtype == "float32c")
     # dominant kernel: the fused main kernels alone, HIP events on the launch stream
     plan = grid._ensure_plan()
     plan.bench_main_kernel(5, stream.cuda_stream)
@@ -149,10 +158,11 @@ def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps):
         mcells=n ** 3 * steps / (ms / 1e3) / 1e6,
         ms_step=ms_step,
         step_frac=2 * half_bytes / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9),
-        roofline={"bound": "hbm", "kernel": f"{dom}<{'double' if itemsize == 8 else 'float'}>",
+        roofline={"bound": "hbm", "kernel": f"{dom}<{ {'float64': 'double', 'float32': 'float'}.get(dtype, dtype)}>",
                   "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": round(achieved / HBM_PEAK_GBS, 4),
-                  "traffic": profiled_traffic(dom, "double" if itemsize == 8 else "float", n, pml),
+                  "traffic": (profiled_traffic(dom, "double" if itemsize == 8 else "float", n, pml)
+                              if dtype != "float32c" else None),
                   "algorithmic_bytes_per_launch": half_bytes,
                   "avg_launch_ms": {k: round(v, 5) for k, v in per_launch.items()}})
 
@@ -220,7 +230,9 @@ def precision_form(args):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / (steps - 1)
         run = {"ms_per_step": round(ms, 4), "Mcells/s": round(np.prod(shape) / (ms / 1e3) / 1e6, 1)}
-        fields = torch.cat([grid.E.reshape(-1), grid.H.reshape(-1)]).double()
+        fields = torch.cat([grid.full_precision("E").reshape(-1), grid.full_precision("H").reshape(-1)])
+        tE, tH = grid._ensure_plan().bench_main_kernel(10, torch.cuda.current_stream().cuda_stream)
+        run["main_kernel_ms"] = {"E": round(tE, 4), "H": round(tH, 4)}
         if ref is None:
             ref = fields
         else:
@@ -305,7 +317,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=50)
-    ap.add_argument("--dtype", default="float64", choices=["float64", "float32"])
+    ap.add_argument("--dtype", default="float64", choices=["float64", "float32", "float32c"])
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--pml", type=int, default=16)
     ap.add_argument("--cpu-steps", type=int, default=2)
@@ -316,9 +328,13 @@ def main():
                     help="plain: the 256^3 headline line; precision: float32 forms vs float64; "
                          "adjoint: gradient/forward time on the differentiation configuration")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement of the plain line")
+    ap.add_argument("--palette", default="auto", choices=["auto", "on", "off"],
+                    help="grid material as a <=256-entry palette (auto: in the compensated form only)")
     ap.add_argument("--adjoint-f64", action="store_true", help="--form adjoint in float64")
     ap.add_argument("--shape", type=int, nargs=3, default=None, help="grid shape for --form precision")
     args = ap.parse_args()
+    global PALETTE
+    PALETTE = args.palette
     if args.gpus != 1:
         sys.stderr.write("bench.py measures one GPU: this build has no multi-GPU path (use --gpus 1)\n")
         sys.exit(2)

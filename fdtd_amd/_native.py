@@ -225,8 +225,9 @@ class Adjoint:
             self._h = None
 
     def run_recorded(self, q0, nsteps, curl_records, plane_records, stream):
-        check(self._L.fdtd_run_recorded(self._h, q0, nsteps, curl_records.data_ptr(), plane_records.data_ptr(),
-                                        stream))
+        check(self._L.fdtd_run_recorded(self._h, q0, nsteps,
+                                        curl_records.data_ptr() if curl_records is not None else None,
+                                        plane_records.data_ptr(), stream))
 
     def run_adjoint(self, nsteps, curl_records, plane_records, stream):
         check(self._L.fdtd_adjoint_run(self._h, nsteps, curl_records.data_ptr(), plane_records.data_ptr(), stream))

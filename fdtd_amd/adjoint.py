@@ -230,7 +230,8 @@ class IntensityGradient:
         for q, m in segments:
             if gradient and mode == "sqrt":
                 checkpoints.append([t.clone() for t in self._state()])
-            adj.run_recorded(q, m, curl, plane, stream)
+            # the curl is needed now only when nothing is recomputed later
+            adj.run_recorded(q, m, curl if mode == "record" else None, plane, stream)
             J += plane[:m].to(torch.float64).square().sum()
         g.time_steps_passed = T
         self.last_info = {"mode": mode, "segment": K, "checkpoints": len(checkpoints)}

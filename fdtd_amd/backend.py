@@ -6,8 +6,9 @@ The package provides the HIP backends only:
     - ``hip.float64``
     - ``hip.float32``  float32 fields (all arithmetic in float32)
     - ``hip.float32c`` float32 fields with an 8-bit compensation term per
-                       value (the float's rounding residual), float64
-                       arithmetic, materials and CPML state
+                       value (the 8 mantissa bits the float drops: 32
+                       significant bits in 5 bytes), float64 arithmetic,
+                       materials and CPML state
 
 Arrays are torch tensors on the HIP device (torch names it "cuda" on ROCm);
 the time stepping runs in hand-written HIP kernels (libfdtd_hip.so). The

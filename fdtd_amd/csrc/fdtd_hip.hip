@@ -103,6 +103,11 @@ struct FdtdPlan {
   const uint8_t* mu_id = nullptr;
   const double* eps_tab = nullptr;
   const double* mu_tab = nullptr;
+  // differentiation record of the next E half step (fdtd_run_recorded):
+  // curl inside rec_db from the main kernel, E inside rec_pb from the tail
+  void* rec_curl = nullptr;
+  void* rec_plane = nullptr;
+  Box rec_db{}, rec_pb{};
 };
 
 struct FdtdAdjoint {
@@ -127,6 +132,14 @@ int check_plan(const FdtdPlan* p) {
 }
 
 int cell_ok(const FdtdPlan* p, int64_t c) { return c >= 0 && c < p->d.cs; }
+
+// log2 of the lanes of a z-row of nz cells, one cell per lane (at most a
+// wavefront)
+int lanes_shift(int nz) {
+  int s = 0;
+  while ((1 << s) < nz && s < 6) ++s;
+  return s;
+}
 
 void free_tables(FdtdPlan* p) {
   // kernels already queued may still read the old tables
@@ -159,7 +172,7 @@ PlainField<float, float> make_field(const FdtdPlan* p, bool H) {
 }
 template <>
 CompField make_field(const FdtdPlan* p, bool H) {
-  return {reinterpret_cast<float*>(H ? p->H : p->E), reinterpret_cast<int8_t*>(H ? p->H_lo : p->E_lo)};
+  return {reinterpret_cast<float*>(H ? p->H : p->E), reinterpret_cast<uint8_t*>(H ? p->H_lo : p->E_lo)};
 }
 template <class MS>
 MS make_mat(const FdtdPlan* p, bool mu);
@@ -192,8 +205,7 @@ MainArgs<FS, MS> main_args(const FdtdPlan* p, bool is_H) {
   a.nfaces = p->n_fused;
   a.objs = p->d_objs;
   a.nobjs = is_H ? 0 : (int)p->objs.size();
-  int bzs = 0;
-  while ((1 << bzs) < p->d.nz && bzs < 6) ++bzs;
+  const int bzs = lanes_shift(p->d.nz);
   a.bz_shift = bzs;
   a.cx = p->cx;
   a.by = std::max(1, p->block_threads >> bzs);
@@ -233,6 +245,10 @@ int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
     a.wave_cap = p->wave_cap;
     a.q = q;
   }
+  if (!is_H && p->rec_curl) {
+    a.rec = reinterpret_cast<typename FS::R*>(p->rec_curl);
+    a.rb = p->rec_db;
+  }
   launch_main<FS, MS, O>(p, a, is_H, st);
   for (const Bnd& b : p->stages) {
     if (b.periodic) {
@@ -258,12 +274,17 @@ int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
   bool any_src = false;
   if (!p->fold_points)
     for (const SrcHost& s : p->srcs) any_src |= (!is_H || s.kind == FDTD_SRC_PLANE);
-  if (any_src || !p->dets.empty()) {
+  const bool rec_plane = !is_H && p->rec_plane;
+  if (any_src || !p->dets.empty() || rec_plane) {
     TailArgs<FS> t{};
     t.d = p->d;
     t.F = a.F;
     t.is_H = is_H ? 1 : 0;
-    t.nsrc = (int)p->srcs.size();
+    t.nsrc = p->fold_points ? 0 : (int)p->srcs.size();
+    if (rec_plane) {
+      t.rec = reinterpret_cast<typename FS::R*>(p->rec_plane);
+      t.rb = p->rec_pb;
+    }
     t.ndet = (int)p->dets.size();
     t.srcs = p->d_srcs;
     t.dets = p->d_dets;
@@ -351,6 +372,11 @@ AdjArgs<T> adj_args(const FdtdAdjoint* a) {
   r.db = a->db;
   r.pb = a->pb;
   r.grad = a->grad;
+  int bzs = 0;
+  while ((1 << bzs) < p->d.nz && bzs < 6) ++bzs;
+  r.bz_shift = bzs;
+  r.cx = p->cx;
+  r.by = std::max(1, p->block_threads >> bzs);
   return r;
 }
 
@@ -358,35 +384,42 @@ template <typename T>
 int adjoint_steps(FdtdAdjoint* a, int64_t nsteps, const T* curl, const T* plane, hipStream_t st) {
   const FdtdPlan* p = a->plan;
   const int64_t nd = box_cells(a->db), np = box_cells(a->pb);
-  const unsigned blocks = (unsigned)((p->d.cs + 255) / 256);
   AdjArgs<T> args = adj_args<T>(a);
+  const int bz = 1 << args.bz_shift, by = args.by;
+  const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
+  const dim3 grid(tiles * (unsigned)((p->d.nx + args.cx - 1) / args.cx)), block(bz * by);
   for (int64_t s = nsteps - 1; s >= 0; --s) {
     args.plane = plane + s * 3 * np;
     args.parity = a->parH;
-    hipLaunchKernelGGL(k_adj_H<T>, dim3(blocks), dim3(256), 0, st, args);
+    hipLaunchKernelGGL(k_adj_H<T>, grid, block, 0, st, args);
     a->parH ^= 1;
     args.curl = curl + s * 3 * nd;
     args.parity = a->parE;
-    hipLaunchKernelGGL(k_adj_E<T>, dim3(blocks), dim3(256), 0, st, args);
+    hipLaunchKernelGGL(k_adj_E<T>, grid, block, 0, st, args);
     a->parE ^= 1;
   }
   FDTD_HIP(hipGetLastError());
   return 0;
 }
 
+// Forward steps with the records written by the main E kernel (curl) and
+// the tail (E inside the objective box); curl may be null (objective only).
 template <typename T>
 int recorded_steps(FdtdAdjoint* a, int64_t q0, int64_t nsteps, T* curl, T* plane, hipStream_t st) {
   FdtdPlan* p = a->plan;
   const int64_t nd = box_cells(a->db), np = box_cells(a->pb);
-  const unsigned blocks = (unsigned)((nd + np + 255) / 256);
-  for (int64_t s = 0; s < nsteps; ++s) {
-    if (int rc = half_dispatch(p, false, q0 + s, st)) return rc;
-    hipLaunchKernelGGL(k_record<T>, dim3(blocks), dim3(256), 0, st, p->d, reinterpret_cast<const T*>(p->E),
-                       reinterpret_cast<const T*>(p->H), a->db, curl + s * 3 * nd, a->pb, plane + s * 3 * np);
-    if (int rc = half_dispatch(p, true, q0 + s, st)) return rc;
+  p->rec_db = a->db;
+  p->rec_pb = a->pb;
+  int rc = 0;
+  for (int64_t s = 0; s < nsteps && rc == 0; ++s) {
+    p->rec_curl = curl ? curl + s * 3 * nd : nullptr;
+    p->rec_plane = plane + s * 3 * np;
+    rc = half_dispatch(p, false, q0 + s, st);
+    p->rec_curl = nullptr;
+    p->rec_plane = nullptr;
+    if (rc == 0) rc = half_dispatch(p, true, q0 + s, st);
   }
-  FDTD_HIP(hipGetLastError());
-  return 0;
+  return rc;
 }
 
 }  // namespace
@@ -574,8 +607,7 @@ int fdtd_plan_finalize(FdtdPlan* p) {
       const int bt = atoi(env);
       if (bt == 256 || bt == 512 || bt == 1024) p->block_threads = bt;
     }
-    int bzs = 0;
-    while ((1 << bzs) < p->d.nz && bzs < 6) ++bzs;
+    const int bzs = lanes_shift(p->d.nz);
     const int bz = 1 << bzs, by = std::max(1, p->block_threads >> bzs);
     const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by);
     int64_t cx = tiles * p->d.nx / 32768;
@@ -793,7 +825,7 @@ int fdtd_run_recorded(FdtdAdjoint* a, int64_t q0, int64_t nsteps, void* curl_rec
   FdtdPlan* p = a->plan;
   if (int rc = adj_check_plan(p)) return rc;
   if (nsteps < 0) return fail(-1, "negative step count");
-  if (nsteps > 0 && (!curl_records || !plane_records)) return fail(-1, "null record buffer");
+  if (nsteps > 0 && !plane_records) return fail(-1, "null record buffer");
   if (int rc = check_step_room(p, q0, nsteps, false, false)) return rc;
   if (int rc = select_device(p)) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);

@@ -15,8 +15,8 @@
 // The kernels are written once over two policies:
 //   field storage  PlainField<S, R>  values stored as S, computed in R
 //                  CompField         float plus an 8-bit compensation term
-//                                    (the rounding residual of the float,
-//                                    scaled to its ulp), computed in double
+//                                    (the next 8 mantissa bits: a 40-bit
+//                                    format), computed in double
 //   material       MatArray<S, R>    per-cell (3, N) array
 //                  MatPalette<R>     one byte per cell indexing <= 256 float64
 //                                    triples
@@ -110,32 +110,32 @@ struct PlainField {
   __device__ __forceinline__ Sample sample(int64_t i) const { return p[i]; }
 };
 
-// 2^e as a double, e in [-1022, 1023]
-__device__ __forceinline__ double pow2(int e) { return __longlong_as_double((long long)(e + 1023) << 52); }
-
-// Exponent E of a float in frexp's convention (h = m 2^E, m in [1/2, 1));
-// zero and subnormals use -125 so that their ulp is 2^(E-24) as well.
-__device__ __forceinline__ int comp_exp(float h) {
-  const int eb = (int)((__float_as_uint(h) >> 23) & 0xffu);
-  return eb == 0 ? -125 : eb - 126;
+// Compensated storage: a value v is kept as the float h holding the top 24
+// significant bits of v rounded to 32 significant bits, plus a byte q with
+// the next 8 bits -- a 40-bit floating-point format whose byte is the
+// compensation term of the float (what rounding v to the float drops).
+// Decoding is exact in double: widen h (its mantissa lands in bits 51..29 of
+// the double) and insert q at bits 28..21. Encoding rounds |v| to nearest at
+// bit 21 by adding 2^20 to the bit pattern (the carry may ripple into the
+// exponent, which is the correct rounding of the magnitude), then splits the
+// result. Relative error of a store: at most 2^-32. Values below the float
+// range (|v| < 2^-126) keep the float's precision only.
+__device__ __forceinline__ double comp_decode(float h, uint8_t q) {
+  const double w = (double)h;
+  const uint32_t lo32 = (uint32_t)__double2loint(w) | ((uint32_t)q << 21);
+  return __hiloint2double(__double2hiint(w), (int)lo32);
 }
-
-// A value v stored as h = (float)v and q = round((v - h) 2^(32-E)), |q| <= 127:
-// the residual |v - h| <= ulp(h)/2 = 2^(E-25) is kept to a quantum of 2^(E-32),
-// about 2^-31 of |h|. Both conversions are exact in double arithmetic.
-__device__ __forceinline__ double comp_decode(float h, int8_t q) {
-  return (double)h + (double)q * pow2(comp_exp(h) - 32);
-}
-__device__ __forceinline__ int8_t comp_encode(double v, float h) {
-  const double s = (v - (double)h) * pow2(32 - comp_exp(h));
-  return (int8_t)(int)fmin(127.0, fmax(-127.0, rint(s)));
+__device__ __forceinline__ float comp_hi(double v, uint8_t& q) {
+  const uint64_t u = (uint64_t)__double_as_longlong(v) + (1ull << 20);
+  q = (uint8_t)((u >> 21) & 0xffu);
+  return (float)__longlong_as_double((long long)(u & ~((1ull << 29) - 1)));
 }
 
 struct CompField {
   using R = double;
   using Sample = float;
   float* hi;
-  int8_t* lo;
+  uint8_t* lo;
   __device__ __forceinline__ double get(int64_t i) const { return comp_decode(hi[i], lo[i]); }
   template <bool NT>
   __device__ __forceinline__ double get_s(int64_t i) const {
@@ -143,20 +143,22 @@ struct CompField {
   }
   template <bool NT>
   __device__ __forceinline__ void put_s(int64_t i, double v) const {
-    const float h = (float)v;
+    uint8_t q;
+    const float h = comp_hi(v, q);
     st_stream<NT>(hi + i, h);
-    st_stream<NT>(lo + i, comp_encode(v, h));
+    st_stream<NT>(lo + i, q);
   }
   __device__ __forceinline__ void put(int64_t i, double v) const {
-    const float h = (float)v;
-    hi[i] = h;
-    lo[i] = comp_encode(v, h);
+    uint8_t q;
+    hi[i] = comp_hi(v, q);
+    lo[i] = q;
   }
   __device__ __forceinline__ void copy(int64_t dst, int64_t src) const {
     hi[dst] = hi[src];
     lo[dst] = lo[src];
   }
-  __device__ __forceinline__ float sample(int64_t i) const { return hi[i]; }
+  // detectors sample the value rounded to float
+  __device__ __forceinline__ float sample(int64_t i) const { return (float)get(i); }
 };
 
 // --------------------------------------------------------------- materials
@@ -298,6 +300,29 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned nb) {
   return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
 }
 
+// Half-open box of cells {x0, x1, y0, y1, z0, z1}; its cells are numbered
+// z fastest.
+struct Box {
+  int32_t x0, x1, y0, y1, z0, z1;
+};
+
+__host__ __device__ __forceinline__ int64_t box_cells(const Box& b) {
+  return (int64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
+}
+
+// index of cell (i,j,k) inside box b, or -1
+__device__ __forceinline__ int64_t box_index(const Box& b, int i, int j, int k) {
+  if (i < b.x0 || i >= b.x1 || j < b.y0 || j >= b.y1 || k < b.z0 || k >= b.z1) return -1;
+  return ((int64_t)(i - b.x0) * (b.y1 - b.y0) + (j - b.y0)) * (b.z1 - b.z0) + (k - b.z0);
+}
+
+// grid cell of box index p
+__device__ __forceinline__ int64_t box_cell(const Box& b, const Dims& d, int64_t p) {
+  const int64_t nz = b.z1 - b.z0, ny = b.y1 - b.y0;
+  const int64_t k = b.z0 + p % nz, j = b.y0 + (p / nz) % ny, i = b.x0 + p / (nz * ny);
+  return (i * d.ny + j) * d.nz + k;
+}
+
 // ------------------------------------------------------------ main kernels
 constexpr int FOLDED_POINTS = 4;
 
@@ -323,6 +348,10 @@ struct MainArgs {
   const double* wave;  // row s = source s
   int64_t wave_cap;
   int64_t q;
+  // differentiation record (E kernel): the curl of this half step inside
+  // box `rb`, (3, cells of rb), when rec is not null
+  R* rec;
+  Box rb;
 };
 
 // Main E kernel. A workgroup owns a (y, z) tile of bz*by columns (z fastest,
@@ -381,6 +410,14 @@ __global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
       curl[0] = (zero + df.zy) - df.yz;
       curl[1] = (zero + df.xz) - df.zx;
       curl[2] = (zero + df.yx) - df.xy;
+      if (a.rec) {
+        const int64_t r = box_index(a.rb, i, j, k);
+        if (r >= 0) {
+          const int64_t nr = box_cells(a.rb);
+#pragma unroll
+          for (int c = 0; c < 3; ++c) a.rec[c * nr + r] = curl[c];
+        }
+      }
       T m[3], kc[3], e[3];
       a.M.template get3<NT>(cell, m);
 #pragma unroll
@@ -556,6 +593,10 @@ struct TailArgs {
   int64_t wave_cap;
   int64_t q;
   int64_t counter;  // half steps of this kind already taken by the plan
+  // differentiation record (E half step): the field inside box `rb` after
+  // the sources, (3, cells of rb), when rec is not null
+  typename FS::R* rec;
+  Box rb;
 };
 
 // Sources in registration order, then detector samples; one workgroup so the
@@ -582,6 +623,13 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
     }
     __syncthreads();
   }
+  if (a.rec) {
+    const int64_t nr = box_cells(a.rb);
+    for (int64_t p = tid; p < 3 * nr; p += blockDim.x) {
+      const int c = (int)(p / nr);
+      a.rec[p] = a.F.get(c * a.d.cs + box_cell(a.rb, a.d, p - c * nr));
+    }
+  }
   for (int t = 0; t < a.ndet; ++t) {
     const DetDev& det = a.dets[t];
     const int64_t slot = a.counter - (a.is_H ? det.slot0H : det.slot0E);
@@ -600,64 +648,18 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+
 // ================================================================ adjoint
 // The step is linear in (E, H, psi) for fixed materials, so reverse-mode
 // differentiation of an objective through T steps applies the transposes of
 // the two half steps in reverse order. The kernels below are those exact
 // transposes for plain storage with every CPML face folded into the step and
 // objects composed into a per-cell affine E update
-//   E' = dcoef * E + gcoef * curl_H(H) + CPML + sources.
-// Sources add constants and drop out of the transpose.
-
-struct Box {
-  int32_t x0, x1, y0, y1, z0, z1;  // half-open
-};
-
-__host__ __device__ __forceinline__ int64_t box_cells(const Box& b) {
-  return (int64_t)(b.x1 - b.x0) * (b.y1 - b.y0) * (b.z1 - b.z0);
-}
-
-// index of cell (i,j,k) inside box b (z fastest), or -1
-__device__ __forceinline__ int64_t box_index(const Box& b, int i, int j, int k) {
-  if (i < b.x0 || i >= b.x1 || j < b.y0 || j >= b.y1 || k < b.z0 || k >= b.z1) return -1;
-  return ((int64_t)(i - b.x0) * (b.y1 - b.y0) + (j - b.y0)) * (b.z1 - b.z0) + (k - b.z0);
-}
-
-__device__ __forceinline__ int64_t box_cell(const Box& b, const Dims& d, int64_t p) {
-  const int64_t nz = b.z1 - b.z0, ny = b.y1 - b.y0;
-  const int64_t k = b.z0 + p % nz, j = b.y0 + (p / nz) % ny, i = b.x0 + p / (nz * ny);
-  return (i * d.ny + j) * d.nz + k;
-}
-
-// After an E half step: the curl of H inside the design box (the curl that
-// half step used; H is unchanged by it) and E inside the objective box,
-// each (3, cells of the box).
-template <typename T>
-__global__ void __launch_bounds__(256) k_record(Dims d, const T* __restrict__ E, const T* __restrict__ H, Box db,
-                                                T* __restrict__ curl_out, Box pb, T* __restrict__ plane_out) {
-  const int64_t nd = box_cells(db), np = box_cells(pb);
-  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
-  if (p < nd) {
-    const int64_t cell = box_cell(db, d, p);
-    const int k = (int)(cell % d.nz), j = (int)((cell / d.nz) % d.ny), i = (int)(cell / sx);
-    const T* Hx = H;
-    const T* Hy = H + cs;
-    const T* Hz = H + 2 * cs;
-    const T zero = T(0);
-    const T zy = j > 0 ? Hz[cell] - Hz[cell - sy] : zero, yz = k > 0 ? Hy[cell] - Hy[cell - 1] : zero;
-    const T xz = k > 0 ? Hx[cell] - Hx[cell - 1] : zero, zx = i > 0 ? Hz[cell] - Hz[cell - sx] : zero;
-    const T yx = i > 0 ? Hy[cell] - Hy[cell - sx] : zero, xy = j > 0 ? Hx[cell] - Hx[cell - sy] : zero;
-    curl_out[p] = (zero + zy) - yz;
-    curl_out[nd + p] = (zero + xz) - zx;
-    curl_out[2 * nd + p] = (zero + yx) - xy;
-  } else if (p < nd + np) {
-    const int64_t q = p - nd;
-    const int64_t cell = box_cell(pb, d, q);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) plane_out[c * np + q] = E[c * cs + cell];
-  }
-}
+//   E' = dcoef * E + gcoef * curl_H(H) + CPML + sources
+// (sources add constants and drop out). They march like the main kernels:
+// the transpose of the H half step gathers backward neighbours (the shape of
+// the E kernel), the transpose of the E half step forward ones (the shape of
+// the H kernel).
 
 // Adjoint CPML state of one face: psi of the E and of the H side, two
 // buffers each, read and written alternately (input `parity`, output
@@ -673,11 +675,11 @@ struct AdjArgs {
   T sc;
   T* lamE;               // dJ/dE, (3, N)
   T* lamH;               // dJ/dH, (3, N)
-  const T* inv_eps;      // grid materials: CPML corrections use S*inv
+  const T* inv_eps;      // grid materials: the CPML terms use S*inv
   const T* inv_mu;
   const T* gcoef;        // (3, N)
   const T* dcoef;        // (3, N) or null (all ones)
-  const FaceDev* faces;  // CPML faces in registration order (b, c profiles)
+  const FaceDev* faces;  // CPML faces (b, c profiles)
   const AdjFaceDev* afaces;
   int32_t nfaces;
   int32_t parity;
@@ -686,6 +688,7 @@ struct AdjArgs {
   double* grad;          // (3, cells of db)
   Box pb;                // objective box: lamE += 2 E
   const T* plane;        // (3, cells of pb) recorded for this step
+  int32_t bz_shift, cx, by;  // launch shape, as for the main kernels
 };
 
 // Slab index of cell (i,j,k) in face fd, or -1.
@@ -697,7 +700,7 @@ __device__ __forceinline__ int face_l(const FaceDev& fd, const Dims& d, int i, i
   return (l >= 0 && l < fd.t) ? l : -1;
 }
 
-// Offset of cell (i,j,k) with slab index l in the face's (slab) layout.
+// Offset of cell (i,j,k), slab index l, in the face's slab layout.
 __device__ __forceinline__ int64_t face_off(const FaceDev& fd, const Dims& d, int i, int j, int k, int l) {
   const int ax = fd.axis;
   const int64_t sny = ax == 1 ? fd.t : d.ny, snz = ax == 2 ? fd.t : d.nz;
@@ -705,174 +708,247 @@ __device__ __forceinline__ int64_t face_off(const FaceDev& fd, const Dims& d, in
   return (li * sny + lj) * snz + lk;
 }
 
+// v[s] += x for a runtime component index s, without indexing registers.
+template <typename T>
+__device__ __forceinline__ void add_at(T& v0, T& v1, T& v2, int s, T x) {
+  v0 = s == 0 ? v0 + x : v0;
+  v1 = s == 1 ? v1 + x : v1;
+  v2 = s == 2 ? v2 + x : v2;
+}
+
 // Transpose of the H half step (grid.py:301-325, boundaries.py:421-431 and
-// 461-487). Forward, per face and psi component q (E component s_q = 3-ax-C_q
-// differenced along the normal):
+// 461-487). Forward, per face normal to ax and psi component q (H component
+// C_q, E component s_q = 3-ax-C_q differenced along ax):
 //   psiH'_q = bH psiH_q + cH(l) (E_s(l+1) - E_s(l))     for l <= t-2
 //   H'      = H - K_H curl_E(E) + sg_q K_H psiH'_q,       K_H = S inv_mu
-// so with w_q = lamPsiH'_q + sg_q K_H lamH (lamH = dJ/dH' = dJ/dH):
+// (phi_H = psi_{C+1} - psi_{C+2} enters H' with a minus sign: sg_q = -1 when
+// ax = C_q + 1 mod 3, +1 otherwise). With Y = K_H lamH (lamH = dJ/dH' =
+// dJ/dH) and w_q = lamPsiH'_q + sg_q Y_{C_q}:
 //   lamPsiH_q = bH w_q
-//   lamE     += -curl_E^T(K_H lamH) + transpose of the forward difference.
+//   lamE      = dcoef lamE - curl_E^T(Y) - cH(l) w_q [at E_s(l)]
+//                                        + cH(l) w_q [at E_s(l+1)]
 // curl_E^T gathers backward neighbours with the masks of curl_E
 // (grid.py:39-49): (curl_E^T Y)_x = ([k>=1] Y_y(k-1) - [k<=nz-2] Y_y(k))
-// - ([j>=1] Y_z(j-1) - [j<=ny-2] Y_z(j)), and cyclically.
-// Two fused extras: lamE is first scaled by dcoef (deferred from the
-// transpose of the following E half step) and, after the update, the
-// objective's 2 E is added inside the objective box.
+// - ([j>=1] Y_z(j-1) - [j<=ny-2] Y_z(j)), and cyclically. The factor dcoef
+// comes from the transpose of the following E half step (k_adj_E leaves it);
+// after the update the objective's 2 E is added inside the objective box.
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_H(AdjArgs<T> a) {
+__global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
   const Dims d = a.d;
-  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= d.cs) return;
-  const int64_t cs = d.cs, sy = d.nz, sx = (int64_t)d.ny * d.nz;
-  const int k = (int)(cell % d.nz), j = (int)((cell / d.nz) % d.ny), i = (int)(cell / sx);
-  const T sc = a.sc;
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
+  const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
+  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  const int lz = threadIdx.x & (bz - 1);
+  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
+  const bool act = k < d.nz && j < d.ny;
+  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const T zero = T(0), sc = a.sc;
   auto Y = [&](int c, int64_t x) { return (sc * a.inv_mu[c * cs + x]) * a.lamH[c * cs + x]; };
-  const T zero = T(0);
-  const T y0 = Y(0, cell), y1 = Y(1, cell), y2 = Y(2, cell);
-  const T t_x = ((k >= 1 ? Y(1, cell - 1) : zero) - (k <= d.nz - 2 ? y1 : zero)) -
-                ((j >= 1 ? Y(2, cell - sy) : zero) - (j <= d.ny - 2 ? y2 : zero));
-  const T t_y = -((k >= 1 ? Y(0, cell - 1) : zero) - (k <= d.nz - 2 ? y0 : zero)) +
-                ((i >= 1 ? Y(2, cell - sx) : zero) - (i <= d.nx - 2 ? y2 : zero));
-  const T t_z = ((j >= 1 ? Y(0, cell - sy) : zero) - (j <= d.ny - 2 ? y0 : zero)) -
-                ((i >= 1 ? Y(1, cell - sx) : zero) - (i <= d.nx - 2 ? y1 : zero));
-  T le[3];
-#pragma unroll
-  for (int c = 0; c < 3; ++c) le[c] = a.dcoef ? a.dcoef[c * cs + cell] * a.lamE[c * cs + cell] : a.lamE[c * cs + cell];
-  le[0] -= t_x;
-  le[1] -= t_y;
-  le[2] -= t_z;
-  for (int f = 0; f < a.nfaces; ++f) {
-    const FaceDev& fd = a.faces[f];
-    const int ax = fd.axis;
-    const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
-    const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
-    // phi_H = psi_{C+1} - psi_{C+2} (indices mod 3) enters H' with a minus
-    // sign: psi_q along ax has weight -K_H when ax = C_q + 1, +K_H otherwise
-    const T sg0 = ax == (c0 + 1) % 3 ? T(-1) : T(1), sg1 = ax == (c1 + 1) % 3 ? T(-1) : T(1);
-    const T* pin = reinterpret_cast<const T*>(a.afaces[f].psiH[a.parity]);
-    T* pout = reinterpret_cast<T*>(a.afaces[f].psiH[1 - a.parity]);
-    const int l = face_l(fd, d, i, j, k);
-    if (l >= 0) {
-      const int64_t off = face_off(fd, d, i, j, k, l);
-      const T w0 = pin[off] + sg0 * Y(c0, cell);
-      const T w1 = pin[fd.slab + off] + sg1 * Y(c1, cell);
-      const T b = (T)fd.bH[l];
-      pout[off] = b * w0;
-      pout[fd.slab + off] = b * w1;
-      if (l <= fd.t - 2) {  // psi'(l) holds c(l) (E(l+1) - E(l))
-        const T c = (T)fd.cH[l];
-        le[s0] -= c * w0;
-        le[s1] -= c * w1;
-      }
-    }
-    // ... and E at this cell is the E(l+1) of the face cell one step back
-    const int64_t st = ax == 0 ? sx : (ax == 1 ? sy : 1);
-    const int ib = ax == 0 ? i - 1 : i, jb = ax == 1 ? j - 1 : j, kb = ax == 2 ? k - 1 : k;
-    if (ib >= 0 && jb >= 0 && kb >= 0) {
-      const int lb = face_l(fd, d, ib, jb, kb);
-      if (lb >= 0 && lb <= fd.t - 2) {
-        const int64_t nb = cell - st;
-        const int64_t off = face_off(fd, d, ib, jb, kb, lb);
-        const T w0 = pin[off] + sg0 * Y(c0, nb);
-        const T w1 = pin[fd.slab + off] + sg1 * Y(c1, nb);
-        const T c = (T)fd.cH[lb];
-        le[s0] += c * w0;
-        le[s1] += c * w1;
-      }
-    }
+  int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
+  T yz_prev = zero, yy_prev = zero;  // Y at plane i-1
+  if (act && i0 > 0) {
+    yz_prev = Y(2, cell - sx);
+    yy_prev = Y(1, cell - sx);
   }
-  const int64_t pp = box_index(a.pb, i, j, k);
-  if (pp >= 0) {
-    const int64_t np = box_cells(a.pb);
-#pragma unroll
-    for (int c = 0; c < 3; ++c) le[c] += T(2) * a.plane[c * np + pp];
+  for (int i = i0; i < i1; ++i, cell += sx) {
+    T y0 = zero, y1 = zero, y2 = zero;
+    if (act) {
+      y0 = Y(0, cell);
+      y1 = Y(1, cell);
+      y2 = Y(2, cell);
+    }
+    T yy_km = __shfl_up(y1, 1);
+    T yx_km = __shfl_up(y0, 1);
+    if (act) {
+      if (lz == 0 && k > 0) {
+        yy_km = Y(1, cell - 1);
+        yx_km = Y(0, cell - 1);
+      }
+      T yz_jm = zero, yx_jm = zero;
+      if (j > 0) {
+        yz_jm = Y(2, cell - sy);
+        yx_jm = Y(0, cell - sy);
+      }
+      const bool km = k >= 1, kp = k <= d.nz - 2, jm = j >= 1, jp = j <= d.ny - 2, im = i >= 1, ip = i <= d.nx - 2;
+      const T t0 = ((km ? yy_km : zero) - (kp ? y1 : zero)) - ((jm ? yz_jm : zero) - (jp ? y2 : zero));
+      const T t1 = ((im ? yz_prev : zero) - (ip ? y2 : zero)) - ((km ? yx_km : zero) - (kp ? y0 : zero));
+      const T t2 = ((jm ? yx_jm : zero) - (jp ? y0 : zero)) - ((im ? yy_prev : zero) - (ip ? y1 : zero));
+      T e0 = a.lamE[cell], e1 = a.lamE[cs + cell], e2 = a.lamE[2 * cs + cell];
+      if (a.dcoef) {
+        e0 = a.dcoef[cell] * e0;
+        e1 = a.dcoef[cs + cell] * e1;
+        e2 = a.dcoef[2 * cs + cell] * e2;
+      }
+      e0 = e0 - t0;
+      e1 = e1 - t1;
+      e2 = e2 - t2;
+      for (int f = 0; f < a.nfaces; ++f) {
+        const FaceDev& fd = a.faces[f];
+        const int ax = fd.axis;
+        const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
+        const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
+        const T sg0 = ax == (c0 + 1) % 3 ? T(-1) : T(1), sg1 = ax == (c1 + 1) % 3 ? T(-1) : T(1);
+        const T* pin = reinterpret_cast<const T*>(a.afaces[f].psiH[a.parity]);
+        T* pout = reinterpret_cast<T*>(a.afaces[f].psiH[1 - a.parity]);
+        const int l = face_l(fd, d, i, j, k);
+        if (l >= 0) {
+          const int64_t off = face_off(fd, d, i, j, k, l);
+          const T w0 = pin[off] + sg0 * (c0 == 0 ? y0 : y1);
+          const T w1 = pin[fd.slab + off] + sg1 * (c1 == 1 ? y1 : y2);
+          const T b = (T)fd.bH[l];
+          pout[off] = b * w0;
+          pout[fd.slab + off] = b * w1;
+          if (l <= fd.t - 2) {
+            const T c = (T)fd.cH[l];
+            add_at(e0, e1, e2, s0, -(c * w0));
+            add_at(e0, e1, e2, s1, -(c * w1));
+          }
+        }
+        // this cell is the E(l+1) of the face cell one step back along ax,
+        // whose Y values this thread already holds
+        const int ib = ax == 0 ? i - 1 : i, jb = ax == 1 ? j - 1 : j, kb = ax == 2 ? k - 1 : k;
+        if (ib >= 0 && jb >= 0 && kb >= 0) {
+          const int lb = face_l(fd, d, ib, jb, kb);
+          if (lb >= 0 && lb <= fd.t - 2) {
+            const int64_t off = face_off(fd, d, ib, jb, kb, lb);
+            const T yb0 = ax == 0 ? yy_prev : (ax == 1 ? yx_jm : yx_km);
+            const T yb1 = ax == 0 ? yz_prev : (ax == 1 ? yz_jm : yy_km);
+            const T w0 = pin[off] + sg0 * yb0;
+            const T w1 = pin[fd.slab + off] + sg1 * yb1;
+            const T c = (T)fd.cH[lb];
+            add_at(e0, e1, e2, s0, c * w0);
+            add_at(e0, e1, e2, s1, c * w1);
+          }
+        }
+      }
+      const int64_t pp = box_index(a.pb, i, j, k);
+      if (pp >= 0) {
+        const int64_t np = box_cells(a.pb);
+        e0 = e0 + T(2) * a.plane[pp];
+        e1 = e1 + T(2) * a.plane[np + pp];
+        e2 = e2 + T(2) * a.plane[2 * np + pp];
+      }
+      a.lamE[cell] = e0;
+      a.lamE[cs + cell] = e1;
+      a.lamE[2 * cs + cell] = e2;
+    }
+    yz_prev = y2;
+    yy_prev = y1;
   }
-#pragma unroll
-  for (int c = 0; c < 3; ++c) a.lamE[c * cs + cell] = le[c];
 }
 
 // Transpose of the E half step (grid.py:275-299, objects.py:118-221,
-// boundaries.py:409-459). Forward, per face and psi component q (H component
-// s_q = 3-ax-C_q differenced along the normal):
+// boundaries.py:409-459). Forward, per face normal to ax and psi component q
+// (E component C_q, H component s_q = 3-ax-C_q differenced along ax):
 //   psiE'_q = bE psiE_q + cE(l) (H_s(l) - H_s(l-1))      for l >= 1
-//   E'      = dcoef E + gcoef curl_H(H) + sg_q K_E psiE'_q, K_E = S inv_eps
-// so with w_q = lamPsiE'_q + sg_q K_E lamE (lamE = dJ/dE'):
+//   E'      = dcoef E + gcoef curl_H(H) + sg_q K_E psiE'_q,  K_E = S inv_eps
+// (phi_E enters E' with a plus sign: sg_q = +1 when ax = C_q + 1 mod 3, -1
+// otherwise; K_E is the grid's, zero inside objects). With Y = gcoef lamE
+// (lamE = dJ/dE') and w_q = lamPsiE'_q + sg_q K_E lamE_{C_q}:
 //   lamPsiE_q = bE w_q
-//   lamH     += curl_H^T(gcoef lamE) + transpose of the backward difference
-//   lamE      = dcoef lamE   (left to the next k_adj_H, which reads lamE
-//                             only at its own cell)
+//   lamH     += curl_H^T(Y) + cE(l) w_q [at H_s(l)] - cE(l) w_q [at H_s(l-1)]
+//   lamE      = dcoef lamE   (left to the next k_adj_H)
+//   grad     += S lamE curl  (dE'/d inv = S curl inside the design box)
 // curl_H^T gathers forward neighbours with the masks of curl_H
 // (grid.py:65-73): (curl_H^T Y)_x = ([k>=1] Y_y - [k<=nz-2] Y_y(k+1))
-// - ([j>=1] Y_z - [j<=ny-2] Y_z(j+1)), and cyclically. Fused: the design
-// gradient grad += S lamE curl (dE'/d inv = S curl inside the design box).
+// - ([j>=1] Y_z - [j<=ny-2] Y_z(j+1)), and cyclically.
 template <typename T>
-__global__ void __launch_bounds__(256) k_adj_E(AdjArgs<T> a) {
+__global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   const Dims d = a.d;
-  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (cell >= d.cs) return;
-  const int64_t cs = d.cs, sy = d.nz, sx = (int64_t)d.ny * d.nz;
-  const int k = (int)(cell % d.nz), j = (int)((cell / d.nz) % d.ny), i = (int)(cell / sx);
-  const T sc = a.sc;
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
+  const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
+  const unsigned tile = gridDim.x - 1 - xcd_tile(blockIdx.x, gridDim.x);
+  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
+  const int lz = threadIdx.x & (bz - 1);
+  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
+  const bool act = k < d.nz && j < d.ny;
+  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const T zero = T(0), sc = a.sc;
   auto Y = [&](int c, int64_t x) { return a.gcoef[c * cs + x] * a.lamE[c * cs + x]; };
   auto KL = [&](int c, int64_t x) { return (sc * a.inv_eps[c * cs + x]) * a.lamE[c * cs + x]; };
-  const T zero = T(0);
-  const T y0 = Y(0, cell), y1 = Y(1, cell), y2 = Y(2, cell);
-  const T t_x = ((k >= 1 ? y1 : zero) - (k <= d.nz - 2 ? Y(1, cell + 1) : zero)) -
-                ((j >= 1 ? y2 : zero) - (j <= d.ny - 2 ? Y(2, cell + sy) : zero));
-  const T t_y = -((k >= 1 ? y0 : zero) - (k <= d.nz - 2 ? Y(0, cell + 1) : zero)) +
-                ((i >= 1 ? y2 : zero) - (i <= d.nx - 2 ? Y(2, cell + sx) : zero));
-  const T t_z = ((j >= 1 ? y0 : zero) - (j <= d.ny - 2 ? Y(0, cell + sy) : zero)) -
-                ((i >= 1 ? y1 : zero) - (i <= d.nx - 2 ? Y(1, cell + sx) : zero));
-  T lh[3] = {a.lamH[cell] + t_x, a.lamH[cs + cell] + t_y, a.lamH[2 * cs + cell] + t_z};
-  for (int f = 0; f < a.nfaces; ++f) {
-    const FaceDev& fd = a.faces[f];
-    const int ax = fd.axis;
-    const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
-    const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
-    // phi_E = psi_{C+1} - psi_{C+2} (indices mod 3) enters E' with a plus
-    // sign: psi_q along ax has weight +K_E when ax = C_q + 1, -K_E otherwise
-    const T sg0 = ax == (c0 + 1) % 3 ? T(1) : T(-1), sg1 = ax == (c1 + 1) % 3 ? T(1) : T(-1);
-    const T* pin = reinterpret_cast<const T*>(a.afaces[f].psiE[a.parity]);
-    T* pout = reinterpret_cast<T*>(a.afaces[f].psiE[1 - a.parity]);
-    const int l = face_l(fd, d, i, j, k);
-    if (l >= 0) {
-      const int64_t off = face_off(fd, d, i, j, k, l);
-      const T w0 = pin[off] + sg0 * KL(c0, cell);
-      const T w1 = pin[fd.slab + off] + sg1 * KL(c1, cell);
-      const T b = (T)fd.bE[l];
-      pout[off] = b * w0;
-      pout[fd.slab + off] = b * w1;
-      if (l >= 1) {  // psi'(l) holds c(l) (H(l) - H(l-1))
-        const T c = (T)fd.cE[l];
-        lh[s0] += c * w0;
-        lh[s1] += c * w1;
-      }
-    }
-    // ... and H at this cell is the H(l-1) of the face cell one step ahead
-    const int64_t st = ax == 0 ? sx : (ax == 1 ? sy : 1);
-    const int iu = ax == 0 ? i + 1 : i, ju = ax == 1 ? j + 1 : j, ku = ax == 2 ? k + 1 : k;
-    if (iu < d.nx && ju < d.ny && ku < d.nz) {
-      const int lu = face_l(fd, d, iu, ju, ku);
-      if (lu >= 1) {
-        const int64_t nu = cell + st;
-        const int64_t off = face_off(fd, d, iu, ju, ku, lu);
-        const T w0 = pin[off] + sg0 * KL(c0, nu);
-        const T w1 = pin[fd.slab + off] + sg1 * KL(c1, nu);
-        const T c = (T)fd.cE[lu];
-        lh[s0] -= c * w0;
-        lh[s1] -= c * w1;
-      }
-    }
+  int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
+  T yz_next = zero, yy_next = zero;  // Y at plane i+1
+  if (act && i1 < d.nx) {
+    yz_next = Y(2, cell + sx);
+    yy_next = Y(1, cell + sx);
   }
+  for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
+    T y0 = zero, y1 = zero, y2 = zero;
+    if (act) {
+      y0 = Y(0, cell);
+      y1 = Y(1, cell);
+      y2 = Y(2, cell);
+    }
+    T yy_kp = __shfl_down(y1, 1);
+    T yx_kp = __shfl_down(y0, 1);
+    if (act) {
+      if (lz == bz - 1 && k < d.nz - 1) {
+        yy_kp = Y(1, cell + 1);
+        yx_kp = Y(0, cell + 1);
+      }
+      T yz_jp = zero, yx_jp = zero;
+      if (j < d.ny - 1) {
+        yz_jp = Y(2, cell + sy);
+        yx_jp = Y(0, cell + sy);
+      }
+      const bool km = k >= 1, kp = k <= d.nz - 2, jm = j >= 1, jp = j <= d.ny - 2, im = i >= 1, ip = i <= d.nx - 2;
+      const T t0 = ((km ? y1 : zero) - (kp ? yy_kp : zero)) - ((jm ? y2 : zero) - (jp ? yz_jp : zero));
+      const T t1 = ((im ? y2 : zero) - (ip ? yz_next : zero)) - ((km ? y0 : zero) - (kp ? yx_kp : zero));
+      const T t2 = ((jm ? y0 : zero) - (jp ? yx_jp : zero)) - ((im ? y1 : zero) - (ip ? yy_next : zero));
+      T h0 = a.lamH[cell] + t0, h1 = a.lamH[cs + cell] + t1, h2 = a.lamH[2 * cs + cell] + t2;
+      for (int f = 0; f < a.nfaces; ++f) {
+        const FaceDev& fd = a.faces[f];
+        const int ax = fd.axis;
+        const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
+        const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
+        const T sg0 = ax == (c0 + 1) % 3 ? T(1) : T(-1), sg1 = ax == (c1 + 1) % 3 ? T(1) : T(-1);
+        const T* pin = reinterpret_cast<const T*>(a.afaces[f].psiE[a.parity]);
+        T* pout = reinterpret_cast<T*>(a.afaces[f].psiE[1 - a.parity]);
+        const int l = face_l(fd, d, i, j, k);
+        if (l >= 0) {
+          const int64_t off = face_off(fd, d, i, j, k, l);
+          const T w0 = pin[off] + sg0 * KL(c0, cell);
+          const T w1 = pin[fd.slab + off] + sg1 * KL(c1, cell);
+          const T b = (T)fd.bE[l];
+          pout[off] = b * w0;
+          pout[fd.slab + off] = b * w1;
+          if (l >= 1) {
+            const T c = (T)fd.cE[l];
+            add_at(h0, h1, h2, s0, c * w0);
+            add_at(h0, h1, h2, s1, c * w1);
+          }
+        }
+        // this cell is the H(l-1) of the face cell one step ahead along ax
+        const int iu = ax == 0 ? i + 1 : i, ju = ax == 1 ? j + 1 : j, ku = ax == 2 ? k + 1 : k;
+        if (iu < d.nx && ju < d.ny && ku < d.nz) {
+          const int lu = face_l(fd, d, iu, ju, ku);
+          if (lu >= 1) {
+            const int64_t st = ax == 0 ? sx : (ax == 1 ? sy : 1);
+            const int64_t off = face_off(fd, d, iu, ju, ku, lu);
+            const T w0 = pin[off] + sg0 * KL(c0, cell + st);
+            const T w1 = pin[fd.slab + off] + sg1 * KL(c1, cell + st);
+            const T c = (T)fd.cE[lu];
+            add_at(h0, h1, h2, s0, -(c * w0));
+            add_at(h0, h1, h2, s1, -(c * w1));
+          }
+        }
+      }
+      a.lamH[cell] = h0;
+      a.lamH[cs + cell] = h1;
+      a.lamH[2 * cs + cell] = h2;
+      const int64_t pd = box_index(a.db, i, j, k);
+      if (pd >= 0) {
+        const int64_t nd = box_cells(a.db);
 #pragma unroll
-  for (int c = 0; c < 3; ++c) a.lamH[c * cs + cell] = lh[c];
-  const int64_t pd = box_index(a.db, i, j, k);
-  if (pd >= 0) {
-    const int64_t nd = box_cells(a.db);
-#pragma unroll
-    for (int c = 0; c < 3; ++c)
-      a.grad[c * nd + pd] += (double)sc * (double)a.lamE[c * cs + cell] * (double)a.curl[c * nd + pd];
+        for (int c = 0; c < 3; ++c)
+          a.grad[c * nd + pd] += (double)sc * (double)a.lamE[c * cs + cell] * (double)a.curl[c * nd + pd];
+      }
+    }
+    yz_next = y2;
+    yy_next = y1;
   }
 }
 

@@ -92,11 +92,12 @@ class Grid:
         self._fill_inverse(self._inv_eps, permittivity)
         self._fill_inverse(self._inv_mu, permeability)
         if self.compensated:
-            # 8-bit compensation terms of the float32 fields (reset whenever the
-            # host writes the fields, see _sync_compensation)
-            self._E_lo = torch.zeros(soa, dtype=torch.int8, device=dev)
-            self._H_lo = torch.zeros(soa, dtype=torch.int8, device=dev)
-            self._field_versions = (self._E._version, self._H._version)
+            # 8 compensation bits per float32 field value. A host write into
+            # grid.E/H[...] changes the float part only: the byte left there
+            # lies below the new float's last bit, so the written value is
+            # kept to float32 precision and every other cell keeps 32 bits.
+            self._E_lo = torch.zeros(soa, dtype=torch.uint8, device=dev)
+            self._H_lo = torch.zeros(soa, dtype=torch.uint8, device=dev)
         # materials read through a <=256-entry float64 palette: "auto" uses one
         # in the compensated mode when the grid has few distinct materials
         self.material_palette = "auto"
@@ -144,6 +145,15 @@ class Grid:
             return
         if not torch.is_tensor(value):
             value = torch.as_tensor(np.asarray(value))
+        if self.compensated:
+            # whole-field assignment keeps 32 significant bits: the float part
+            # and the next 8 mantissa bits (rounded at bit 21 of the double)
+            v = value.to(device=dst.device, dtype=torch.float64).expand_as(view)
+            bits = v.contiguous().view(torch.int64) + (1 << 20)
+            lo = _soa_view(self._E_lo if dst is self._E else self._H_lo)
+            lo.copy_(((bits >> 21) & 0xFF).to(torch.uint8))
+            view.copy_((bits & ~((1 << 29) - 1)).view(torch.float64).to(torch.float32))
+            return
         view.copy_(value.to(device=dst.device, dtype=dst.dtype).expand_as(view))
 
     @property
@@ -157,6 +167,18 @@ class Grid:
     @property
     def H(self):
         return _soa_view(self._H)
+
+    def full_precision(self, field="E"):
+        """The field ("E" or "H") as a float64 (Nx, Ny, Nz, 3) tensor. On
+        hip.float32c this decodes the 40-bit values (the float plus its 8
+        compensation bits, inserted below the float's mantissa) that E/H --
+        the float part alone -- round away; elsewhere it is a cast."""
+        soa = self._E if field == "E" else self._H
+        if not self.compensated:
+            return _soa_view(soa.to(torch.float64))
+        lo = self._E_lo if field == "E" else self._H_lo
+        bits = soa.to(torch.float64).view(torch.int64) | (lo.to(torch.int64) << 21)
+        return _soa_view(bits.view(torch.float64))
 
     @H.setter
     def H(self, value):
@@ -270,16 +292,6 @@ class Grid:
         self._palette_key = key
         return self._palette
 
-    def _sync_compensation(self):
-        """Host writes to the float32 fields invalidate their compensation terms."""
-        if not self.compensated:
-            return
-        if self._E._version != self._field_versions[0]:
-            self._E_lo.zero_()
-        if self._H._version != self._field_versions[1]:
-            self._H_lo.zero_()
-        self._field_versions = (self._E._version, self._H._version)
-
     def _current_plan_key(self):
         key = [len(self.boundaries), len(self.objects), len(self.sources), len(self.detectors), self.courant_number,
                self._palette_wanted()]
@@ -335,7 +347,6 @@ class Grid:
         self._plan.set_waveforms(self._wave)
 
     def _prepare(self, nsteps, e_steps, h_steps):
-        self._sync_compensation()
         plan = self._ensure_plan()
         grown = False
         for det in self.detectors:
@@ -415,6 +426,9 @@ class Grid:
         (grid.py:327-331)"""
         self.H *= 0.0
         self.E *= 0.0
+        if self.compensated:
+            self._E_lo.zero_()
+            self._H_lo.zero_()
         self.time_steps_passed *= 0
 
     def add_source(self, name, source):

@@ -56,9 +56,10 @@ int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number,
 
 int fdtd_plan_destroy(FdtdPlan* plan);
 
-/* FDTD_F32C only, required: the 8-bit compensation terms of E and H, int8
- * (3, nx, ny, nz) each. The stored value is E + q * 2^(e-32), e the binary
- * exponent of the float E in frexp's convention. */
+/* FDTD_F32C only, required: the 8-bit compensation terms of E and H, uint8
+ * (3, nx, ny, nz) each. A value is the float E (its top 24 significant bits)
+ * extended by the 8 mantissa bits that follow, held in the byte: 32
+ * significant bits in all; a zero byte leaves the float's value unchanged. */
 int fdtd_plan_set_compensation(FdtdPlan* plan, void* E_lo, void* H_lo);
 
 /* Optional, any dtype: read the grid materials through a palette instead of
@@ -170,7 +171,8 @@ int fdtd_adjoint_destroy(FdtdAdjoint* adj);
 /* nsteps forward steps from time index q0 through the adjoint's plan. After
  * the E half step of local step s: curl_records[s] = curl H inside the design
  * box (3, cells) and plane_records[s] = E inside the objective box (3, cells),
- * plan dtype. */
+ * plan dtype. curl_records may be NULL (only the objective box is recorded).
+ * The records are written by the step kernels themselves (no extra launch). */
 int fdtd_run_recorded(FdtdAdjoint* adj, int64_t q0, int64_t nsteps, void* curl_records, void* plane_records,
                       void* stream);
 
