@@ -306,6 +306,82 @@ def adjoint_measure(dtype="float32", checkpoints=("sqrt", "record"), reps=3):
     return out
 
 
+def batch_grids(fdtd, torch, count=64, seed=0, pml=10):
+    """The batching configuration: `count` grids with edges drawn uniformly
+    from 64-192 cells, each with a per-voxel permittivity of smoothed Gaussian
+    noise mapped to [1, 12] (noise on a grid 8x coarser, trilinearly
+    interpolated), 10-cell PML on all faces and a pulsed 1550 nm point source
+    at a random interior position."""
+    import torch.nn.functional as Fn
+
+    rng = np.random.default_rng(seed)
+    grids = []
+    for _ in range(count):
+        shape = tuple(int(v) for v in rng.integers(64, 193, size=3))
+        coarse = torch.from_numpy(rng.normal(size=tuple(n // 8 + 2 for n in shape)))
+        fine = Fn.interpolate(coarse[None, None], size=shape, mode="trilinear", align_corners=True)[0, 0]
+        eps = 1.0 + 11.0 * (fine - fine.min()) / (fine.max() - fine.min())
+        grid = fdtd.Grid(shape, grid_spacing=50e-9, permittivity=eps[..., None], courant_number=3 ** -0.5)
+        for k in PML_KEYS(pml):
+            grid[k] = fdtd.PML()
+        p = [int(rng.integers(pml, n - pml)) for n in shape]
+        grid[p[0], p[1], p[2]] = fdtd.PointSource(period=1550e-9 / 299792458.0, pulse=True)
+        grids.append(grid)
+    return grids
+
+
+def batch_measure(steps=2000, seq_steps=200, count=64):
+    """Aggregate throughput of the batched step (autotuned launch shape, HIP
+    graph replays) on the batching configuration, beside the same grids run
+    one after another through the single-grid path."""
+    import torch
+
+    import fdtd_amd as fdtd
+    from fdtd_amd.batch import GridBatch
+
+    fdtd.set_backend("hip.float32")
+    grids = batch_grids(fdtd, torch, count)
+    cells = sum(g.Nx * g.Ny * g.Nz for g in grids)
+    slab_cells = sum(2 * 10 * (g.Ny * g.Nz + g.Nx * g.Nz + g.Nx * g.Ny) for g in grids)
+    step_bytes = 2 * (cells * 12 * 4 + slab_cells * 4 * 4)
+    batch = GridBatch(grids, graph_steps=100)
+    tune = batch.autotune(steps=10)
+    stream = torch.cuda.current_stream()
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        return e0.elapsed_time(e1)
+
+    batch.run(100)  # graph capture and warm-up
+    ms = timed(lambda: batch.run(steps))
+    ms_direct = timed(lambda: batch.run(seq_steps, graph=False)) / seq_steps
+    ms_seq = timed(lambda: [g.run(seq_steps, progress_bar=False) for g in grids]) / seq_steps
+    ms_step = ms / steps
+    achieved = step_bytes / (ms_step / 1e3) / 1e9
+    out = {"metric": "batched Mcells/s (64 uneven fp32 grids)", "unit": "Mcells/s", "dtype": "f32",
+           "value": round(cells / (ms_step / 1e3) / 1e6, 1),
+           "config": {"workload": f"batching: {count} grids, edges U[64,192], smoothed-noise eps in [1,12], "
+                                  "10-cell PML x6, pulsed point source", "cells": cells, "steps": steps},
+           "ms_per_step": round(ms_step, 4),
+           "graph_vs_direct": round(ms_direct / ms_step, 3),
+           "sequential_single_grid": {"Mcells/s": round(cells / (ms_seq / 1e3) / 1e6, 1),
+                                      "ms_per_step": round(ms_seq, 4), "steps": seq_steps},
+           "speedup_vs_sequential": round(ms_seq / ms_step, 3),
+           "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                        "frac": round(achieved / HBM_PEAK_GBS, 4), "algorithmic_bytes_per_step": step_bytes},
+           "launch": {"block_threads": batch.launch[0], "cx": batch.launch[1], "workgroups": batch.launch[2]},
+           "autotune_ms_per_step": {f"{k[0]}x{k[1]}x{k[2]}": round(v, 4) for k, v in sorted(tune.items())}}
+    del batch, grids
+    torch.cuda.empty_cache()
+    return out
+
+
 def adjoint_form(args):
     out = adjoint_measure("float64" if args.adjoint_f64 else "float32")
     out["higher_is_better"] = False
@@ -324,10 +400,12 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-fp32", action="store_true", help="skip the float32 measurement")
-    ap.add_argument("--form", default="plain", choices=["plain", "precision", "adjoint"],
+    ap.add_argument("--form", default="plain", choices=["plain", "precision", "adjoint", "batch"],
                     help="plain: the 256^3 headline line; precision: float32 forms vs float64; "
-                         "adjoint: gradient/forward time on the differentiation configuration")
+                         "adjoint: gradient/forward time on the differentiation configuration; "
+                         "batch: 64 uneven grids stepped together")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint measurement of the plain line")
+    ap.add_argument("--no-batch", action="store_true", help="skip the batching measurement of the plain line")
     ap.add_argument("--palette", default="auto", choices=["auto", "on", "off"],
                     help="grid material as a <=256-entry palette (auto: in the compensated form only)")
     ap.add_argument("--adjoint-f64", action="store_true", help="--form adjoint in float64")
@@ -343,6 +421,11 @@ def main():
         return precision_form(args)
     if args.form == "adjoint":
         return adjoint_form(args)
+    if args.form == "batch":
+        out = batch_measure()
+        out["higher_is_better"] = True
+        print(json.dumps(out), flush=True)
+        return
 
     import torch
 
@@ -386,6 +469,8 @@ def main():
                        "speedup_vs_f64": round(other["mcells"] / main["mcells"], 3)}
     if not args.no_adjoint:
         out["adjoint"] = adjoint_measure("float32")
+    if not args.no_batch:
+        out["batch"] = batch_measure()
     print(json.dumps(out), flush=True)
 
 

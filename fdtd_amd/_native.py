@@ -43,6 +43,12 @@ _SIGNATURES = {
     "fdtd_bench_main_kernel": ([_vp, _c_i64, _vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
                                _c_int),
     "fdtd_bench_copy": ([_vp, _vp, _c_i64, _c_i64, _vp, ctypes.POINTER(ctypes.c_float)], _c_int),
+    "fdtd_batch_create": ([_c_int, ctypes.POINTER(_vp), ctypes.POINTER(_vp)], _c_int),
+    "fdtd_batch_destroy": ([_vp], _c_int),
+    "fdtd_batch_set_launch": ([_vp, _c_int, _c_int, _c_int], _c_int),
+    "fdtd_batch_work_items": ([_vp, _pi64], _c_int),
+    "fdtd_batch_run": ([_vp, _c_i64, _c_i64, _vp], _c_int),
+    "fdtd_batch_run_graph": ([_vp, _c_i64, _c_i64, _c_i64, _vp], _c_int),
     "fdtd_adjoint_psi_size": ([_vp, _pi64], _c_int),
     "fdtd_adjoint_create": ([_vp, _vp, _vp, _vp, _vp, _vp, _pi32, _pi32, _vp, ctypes.POINTER(_vp)], _c_int),
     "fdtd_adjoint_destroy": ([_vp], _c_int),
@@ -231,3 +237,36 @@ class Adjoint:
 
     def run_adjoint(self, nsteps, curl_records, plane_records, stream):
         check(self._L.fdtd_adjoint_run(self._h, nsteps, curl_records.data_ptr(), plane_records.data_ptr(), stream))
+
+
+class Batch:
+    """Owner of one FdtdBatch* over finalized Plans (kept alive here)."""
+
+    def __init__(self, plans):
+        L = plans[0]._L
+        self._L = L
+        arr = (_vp * len(plans))(*[p._h.value for p in plans])
+        h = _vp()
+        check(L.fdtd_batch_create(len(plans), arr, ctypes.byref(h)))
+        self._h = h
+        self._plans = list(plans)
+
+    def __del__(self):
+        h = getattr(self, "_h", None)
+        if h is not None and h.value:
+            self._L.fdtd_batch_destroy(h)
+            self._h = None
+
+    def set_launch(self, block_threads, cx, workgroups):
+        check(self._L.fdtd_batch_set_launch(self._h, int(block_threads), int(cx), int(workgroups)))
+
+    def work_items(self):
+        n = ctypes.c_int64()
+        check(self._L.fdtd_batch_work_items(self._h, ctypes.byref(n)))
+        return n.value
+
+    def run(self, q0, nsteps, stream):
+        check(self._L.fdtd_batch_run(self._h, q0, nsteps, stream))
+
+    def run_graph(self, q0, nsteps, steps_per_graph, stream):
+        check(self._L.fdtd_batch_run_graph(self._h, q0, nsteps, steps_per_graph, stream))

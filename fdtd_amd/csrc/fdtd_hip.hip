@@ -110,6 +110,24 @@ struct FdtdPlan {
   Box rec_db{}, rec_pb{};
 };
 
+struct FdtdBatch {
+  std::vector<FdtdPlan*> plans;
+  int block_threads = 256, cx = 1, groups = 0;  // launch shape (groups 0: one per work item)
+  void* dev = nullptr;        // sims_E, sims_H, first, q
+  void* sims_E = nullptr;
+  void* sims_H = nullptr;
+  int64_t* first = nullptr;
+  int32_t* item_sim = nullptr;
+  int64_t* q = nullptr;
+  int64_t total = 0;
+  bool objs = false;          // some grid has objects: the E kernel variant with them
+  // captured steps
+  hipStream_t gstream = nullptr;
+  hipEvent_t ev_in = nullptr, ev_out = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  int64_t gsteps = 0;
+};
+
 struct FdtdAdjoint {
   FdtdPlan* plan = nullptr;
   const void* gcoef = nullptr;
@@ -218,20 +236,25 @@ void launch_main(const FdtdPlan* p, const MainArgs<FS, MS>& a, bool is_H, hipStr
   const int bz = 1 << a.bz_shift, by = a.by;
   const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
   const dim3 grid(tiles * (unsigned)((p->d.nx + a.cx - 1) / a.cx)), block(bz * by);
-  if (p->nontemporal) {
+  if (a.rec) {  // recording forward pass of a gradient (plain storage)
+    if (a.nobjs)
+      hipLaunchKernelGGL((k_main_E<FS, MS, O, true, true, true>), grid, block, 0, st, a);
+    else
+      hipLaunchKernelGGL((k_main_E<FS, MS, O, false, true, true>), grid, block, 0, st, a);
+  } else if (p->nontemporal) {
     if (is_H)
       hipLaunchKernelGGL((k_main_H<FS, MS, true>), grid, block, 0, st, a);
     else if (a.nobjs)
-      hipLaunchKernelGGL((k_main_E<FS, MS, O, true, true>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((k_main_E<FS, MS, O, true, true, false>), grid, block, 0, st, a);
     else
-      hipLaunchKernelGGL((k_main_E<FS, MS, O, false, true>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((k_main_E<FS, MS, O, false, true, false>), grid, block, 0, st, a);
   } else {
     if (is_H)
       hipLaunchKernelGGL((k_main_H<FS, MS, false>), grid, block, 0, st, a);
     else if (a.nobjs)
-      hipLaunchKernelGGL((k_main_E<FS, MS, O, true, false>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((k_main_E<FS, MS, O, true, false, false>), grid, block, 0, st, a);
     else
-      hipLaunchKernelGGL((k_main_E<FS, MS, O, false, false>), grid, block, 0, st, a);
+      hipLaunchKernelGGL((k_main_E<FS, MS, O, false, false, false>), grid, block, 0, st, a);
   }
 }
 
@@ -420,6 +443,124 @@ int recorded_steps(FdtdAdjoint* a, int64_t q0, int64_t nsteps, T* curl, T* plane
     if (rc == 0) rc = half_dispatch(p, true, q0 + s, st);
   }
   return rc;
+}
+
+// ------------------------------------------------------------------ batch
+int batch_check_plan(const FdtdPlan* p, const FdtdPlan* p0) {
+  if (int rc = check_plan(p)) return rc;
+  if (p->dtype != p0->dtype) return fail(-1, "batched grids must share one storage type");
+  if ((p->eps_id != nullptr) != (p0->eps_id != nullptr)) return fail(-1, "batched grids must all use a palette or none");
+  if (!p->stages.empty()) return fail(-1, "batched grids cannot have periodic boundaries");
+  if (!p->dets.empty()) return fail(-1, "batched grids cannot have detectors");
+  if (!p->srcs.empty() && !p->fold_points) return fail(-1, "batched grids may have at most 4 point sources only");
+  return 0;
+}
+
+void batch_free_graph(FdtdBatch* b) {
+  if (b->gexec) (void)hipGraphExecDestroy(b->gexec);
+  b->gexec = nullptr;
+  b->gsteps = 0;
+}
+
+// (Re)build the per-grid argument tables for the current launch shape.
+int batch_upload(FdtdBatch* b) {
+  FdtdPlan* p0 = b->plans[0];
+  if (int rc = select_device(p0)) return rc;
+  batch_free_graph(b);
+  if (b->dev) {
+    (void)hipDeviceSynchronize();
+    (void)hipFree(b->dev);
+    b->dev = nullptr;
+  }
+  const size_t n = b->plans.size();
+  std::vector<int64_t> first(n + 1, 0);
+  b->objs = false;
+  return with_policies(p0, [&]<class FS, class MS, typename O>() -> int {
+    using MA = MainArgs<FS, MS>;
+    std::vector<MA> sE(n), sH(n);
+    for (size_t i = 0; i < n; ++i) {
+      FdtdPlan* p = b->plans[i];
+      MA aE = main_args<FS, MS>(p, false), aH = main_args<FS, MS>(p, true);
+      const int bzs = lanes_shift(p->d.nz);
+      for (MA* a : {&aE, &aH}) {
+        a->cx = std::max(1, std::min(b->cx, p->d.nx));
+        a->by = std::max(1, b->block_threads >> bzs);
+      }
+      if (p->fold_points) {
+        aE.npts = (int32_t)p->srcs.size();
+        for (int s = 0; s < aE.npts; ++s) aE.pt_cell[s] = p->srcs[s].cells[0];
+        aE.wave = p->wave;
+        aE.wave_cap = p->wave_cap;
+      }
+      b->objs = b->objs || aE.nobjs > 0;
+      sE[i] = aE;
+      sH[i] = aH;
+      const int bz = 1 << bzs;
+      const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + aE.by - 1) / aE.by) *
+                            ((p->d.nx + aE.cx - 1) / aE.cx);
+      first[i + 1] = first[i] + tiles;
+    }
+    b->total = first[n];
+    if (b->total >= (int64_t)1 << 31) return fail(-1, "batch has too many work items");
+    std::vector<int32_t> item_sim(b->total);
+    for (size_t i = 0; i < n; ++i)
+      for (int64_t t = first[i]; t < first[i + 1]; ++t) item_sim[t] = (int32_t)i;
+    const size_t oE = 0, oH = align_up(n * sizeof(MA)), oF = oH + align_up(n * sizeof(MA)),
+                 oQ = oF + align_up((n + 1) * sizeof(int64_t)), oI = oQ + 256,
+                 total = oI + align_up(b->total * sizeof(int32_t));
+    std::vector<unsigned char> host(total, 0);
+    std::memcpy(host.data() + oE, sE.data(), n * sizeof(MA));
+    std::memcpy(host.data() + oH, sH.data(), n * sizeof(MA));
+    std::memcpy(host.data() + oF, first.data(), (n + 1) * sizeof(int64_t));
+    std::memcpy(host.data() + oI, item_sim.data(), b->total * sizeof(int32_t));
+    FDTD_HIP(hipMalloc(&b->dev, total));
+    FDTD_HIP(hipMemcpy(b->dev, host.data(), total, hipMemcpyHostToDevice));
+    char* base = reinterpret_cast<char*>(b->dev);
+    b->sims_E = base + oE;
+    b->sims_H = base + oH;
+    b->first = reinterpret_cast<int64_t*>(base + oF);
+    b->q = reinterpret_cast<int64_t*>(base + oQ);
+    b->item_sim = reinterpret_cast<int32_t*>(base + oI);
+    return 0;
+  });
+}
+
+int batch_groups(const FdtdBatch* b) {
+  if (b->groups > 0 && b->groups < b->total) return b->groups;
+  return (int)b->total;  // one workgroup per work item
+}
+
+// nsteps batched steps on stream st (the time index already set on the device)
+int batch_launch(FdtdBatch* b, int64_t nsteps, hipStream_t st) {
+  const dim3 grid((unsigned)batch_groups(b)), block((unsigned)b->block_threads);
+  return with_policies(b->plans[0], [&]<class FS, class MS, typename O>() -> int {
+    BatchArgs<FS, MS> aE{reinterpret_cast<const MainArgs<FS, MS>*>(b->sims_E), b->first, b->item_sim, b->total,
+                         b->q};
+    BatchArgs<FS, MS> aH = aE;
+    aH.sims = reinterpret_cast<const MainArgs<FS, MS>*>(b->sims_H);
+    for (int64_t s = 0; s < nsteps; ++s) {
+      if (b->objs)
+        hipLaunchKernelGGL((k_batch_E<FS, MS, O, true, true>), grid, block, 0, st, aE);
+      else
+        hipLaunchKernelGGL((k_batch_E<FS, MS, O, false, true>), grid, block, 0, st, aE);
+      hipLaunchKernelGGL((k_batch_H<FS, MS, true>), grid, block, 0, st, aH);
+    }
+    FDTD_HIP(hipGetLastError());
+    return 0;
+  });
+}
+
+int batch_room(FdtdBatch* b, int64_t q0, int64_t nsteps) {
+  for (FdtdPlan* p : b->plans)
+    if (int rc = check_step_room(p, q0, nsteps, false, false)) return rc;
+  return 0;
+}
+
+void batch_advance(FdtdBatch* b, int64_t nsteps) {
+  for (FdtdPlan* p : b->plans) {
+    p->nE += nsteps;
+    p->nH += nsteps;
+  }
 }
 
 }  // namespace
@@ -849,6 +990,107 @@ int fdtd_adjoint_run(FdtdAdjoint* a, int64_t nsteps, const void* curl_records, c
                                  reinterpret_cast<const double*>(plane_records), st);
   return adjoint_steps<float>(a, nsteps, reinterpret_cast<const float*>(curl_records),
                               reinterpret_cast<const float*>(plane_records), st);
+}
+
+int fdtd_batch_create(int n, FdtdPlan* const* plans, FdtdBatch** out) {
+  if (!out) return fail(-1, "null output pointer");
+  *out = nullptr;
+  if (n < 1 || !plans) return fail(-1, "a batch needs at least one plan");
+  for (int i = 0; i < n; ++i)
+    if (int rc = batch_check_plan(plans[i], plans[0])) return rc;
+  FdtdBatch* b = new FdtdBatch();
+  b->plans.assign(plans, plans + n);
+  if (int rc = batch_upload(b)) {
+    delete b;
+    return rc;
+  }
+  *out = b;
+  return 0;
+}
+
+int fdtd_batch_destroy(FdtdBatch* b) {
+  if (!b) return 0;
+  (void)hipDeviceSynchronize();
+  batch_free_graph(b);
+  if (b->gstream) (void)hipStreamDestroy(b->gstream);
+  if (b->ev_in) (void)hipEventDestroy(b->ev_in);
+  if (b->ev_out) (void)hipEventDestroy(b->ev_out);
+  if (b->dev) (void)hipFree(b->dev);
+  delete b;
+  return 0;
+}
+
+int fdtd_batch_set_launch(FdtdBatch* b, int block_threads, int cx, int workgroups) {
+  if (!b) return fail(-1, "null batch");
+  if (block_threads != 256 && block_threads != 512 && block_threads != 1024)
+    return fail(-1, "block_threads must be 256, 512 or 1024");
+  if (cx < 1 || workgroups < 0) return fail(-1, "bad launch shape");
+  b->block_threads = block_threads;
+  b->cx = cx;
+  b->groups = workgroups;
+  return batch_upload(b);
+}
+
+int fdtd_batch_work_items(const FdtdBatch* b, int64_t* items) {
+  if (!b || !items) return fail(-1, "null argument");
+  *items = b->total;
+  return 0;
+}
+
+int fdtd_batch_run(FdtdBatch* b, int64_t q0, int64_t nsteps, void* stream) {
+  if (!b) return fail(-1, "null batch");
+  if (nsteps < 0) return fail(-1, "negative step count");
+  if (int rc = batch_room(b, q0, nsteps)) return rc;
+  if (int rc = select_device(b->plans[0])) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_set_time, dim3(1), dim3(64), 0, st, b->q, q0);
+  if (int rc = batch_launch(b, nsteps, st)) return rc;
+  batch_advance(b, nsteps);
+  return 0;
+}
+
+int fdtd_batch_run_graph(FdtdBatch* b, int64_t q0, int64_t nsteps, int64_t steps_per_graph, void* stream) {
+  if (!b) return fail(-1, "null batch");
+  if (nsteps < 0 || steps_per_graph < 1) return fail(-1, "bad step counts");
+  if (int rc = batch_room(b, q0, nsteps)) return rc;
+  if (int rc = select_device(b->plans[0])) return rc;
+  hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
+  if (!b->gstream) {
+    FDTD_HIP(hipStreamCreateWithFlags(&b->gstream, hipStreamNonBlocking));
+    FDTD_HIP(hipEventCreateWithFlags(&b->ev_in, hipEventDisableTiming));
+    FDTD_HIP(hipEventCreateWithFlags(&b->ev_out, hipEventDisableTiming));
+  }
+  if (!b->gexec || b->gsteps != steps_per_graph) {
+    batch_free_graph(b);
+    hipGraph_t g = nullptr;
+    FDTD_HIP(hipStreamBeginCapture(b->gstream, hipStreamCaptureModeThreadLocal));
+    const int rc = batch_launch(b, steps_per_graph, b->gstream);
+    const hipError_t e = hipStreamEndCapture(b->gstream, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    if (e != hipSuccess) return fail(-3, std::string("graph capture: ") + hipGetErrorString(e));
+    const hipError_t ei = hipGraphInstantiate(&b->gexec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ei != hipSuccess) {
+      b->gexec = nullptr;
+      return fail(-3, std::string("graph instantiate: ") + hipGetErrorString(ei));
+    }
+    b->gsteps = steps_per_graph;
+  }
+  // the caller's stream -> the batch stream -> back
+  hipLaunchKernelGGL(k_set_time, dim3(1), dim3(64), 0, caller, b->q, q0);
+  FDTD_HIP(hipEventRecord(b->ev_in, caller));
+  FDTD_HIP(hipStreamWaitEvent(b->gstream, b->ev_in, 0));
+  const int64_t reps = nsteps / steps_per_graph, rest = nsteps % steps_per_graph;
+  for (int64_t r = 0; r < reps; ++r) FDTD_HIP(hipGraphLaunch(b->gexec, b->gstream));
+  if (rest)
+    if (int rc = batch_launch(b, rest, b->gstream)) return rc;
+  FDTD_HIP(hipEventRecord(b->ev_out, b->gstream));
+  FDTD_HIP(hipStreamWaitEvent(caller, b->ev_out, 0));
+  batch_advance(b, nsteps);
+  return 0;
 }
 
 int fdtd_bench_main_kernel(FdtdPlan* p, int64_t reps, void* stream, float* ms_E, float* ms_H) {

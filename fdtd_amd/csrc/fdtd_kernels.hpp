@@ -83,15 +83,27 @@ struct Dims {
 // Loads and stores of the read-once streams (the updated field and the
 // material); NT marks them non-temporal so the L2 keeps the other field,
 // whose planes and rows are re-read as halos.
+// Every array lives in global memory. Pointers read from tables in device
+// memory (the per-grid arguments of the batched kernels, the face, object,
+// source and detector tables) carry no address space, and accesses through
+// them would be generic (flat) ones; gp() states the global space, which
+// gives global loads with a scalar base.
+template <typename T>
+using gptr = __attribute__((address_space(1))) T*;
+template <typename T>
+__device__ __forceinline__ gptr<T> gp(T* p) {
+  return (gptr<T>)p;
+}
+
 template <bool NT, typename T>
 __device__ __forceinline__ T ld_stream(const T* p) {
-  if constexpr (NT) return __builtin_nontemporal_load(p);
-  else return *p;
+  if constexpr (NT) return __builtin_nontemporal_load(gp(p));
+  else return *gp(p);
 }
 template <bool NT, typename T>
 __device__ __forceinline__ void st_stream(T* p, T v) {
-  if constexpr (NT) __builtin_nontemporal_store(v, p);
-  else *p = v;
+  if constexpr (NT) __builtin_nontemporal_store(v, gp(p));
+  else *gp(p) = v;
 }
 
 // ------------------------------------------------------------ field storage
@@ -100,14 +112,14 @@ struct PlainField {
   using R = R_;
   using Sample = S;
   S* p;
-  __device__ __forceinline__ R get(int64_t i) const { return (R)p[i]; }
+  __device__ __forceinline__ R get(int64_t i) const { return (R)gp(p)[i]; }
   template <bool NT>
   __device__ __forceinline__ R get_s(int64_t i) const { return (R)ld_stream<NT>(p + i); }
   template <bool NT>
   __device__ __forceinline__ void put_s(int64_t i, R v) const { st_stream<NT>(p + i, (S)v); }
-  __device__ __forceinline__ void put(int64_t i, R v) const { p[i] = (S)v; }
-  __device__ __forceinline__ void copy(int64_t dst, int64_t src) const { p[dst] = p[src]; }
-  __device__ __forceinline__ Sample sample(int64_t i) const { return p[i]; }
+  __device__ __forceinline__ void put(int64_t i, R v) const { gp(p)[i] = (S)v; }
+  __device__ __forceinline__ void copy(int64_t dst, int64_t src) const { gp(p)[dst] = gp(p)[src]; }
+  __device__ __forceinline__ Sample sample(int64_t i) const { return gp(p)[i]; }
 };
 
 // Compensated storage: a value v is kept as the float h holding the top 24
@@ -136,7 +148,7 @@ struct CompField {
   using Sample = float;
   float* hi;
   uint8_t* lo;
-  __device__ __forceinline__ double get(int64_t i) const { return comp_decode(hi[i], lo[i]); }
+  __device__ __forceinline__ double get(int64_t i) const { return comp_decode(gp(hi)[i], gp(lo)[i]); }
   template <bool NT>
   __device__ __forceinline__ double get_s(int64_t i) const {
     return comp_decode(ld_stream<NT>(hi + i), ld_stream<NT>(lo + i));
@@ -150,12 +162,12 @@ struct CompField {
   }
   __device__ __forceinline__ void put(int64_t i, double v) const {
     uint8_t q;
-    hi[i] = comp_hi(v, q);
-    lo[i] = q;
+    gp(hi)[i] = comp_hi(v, q);
+    gp(lo)[i] = q;
   }
   __device__ __forceinline__ void copy(int64_t dst, int64_t src) const {
-    hi[dst] = hi[src];
-    lo[dst] = lo[src];
+    gp(hi)[dst] = gp(hi)[src];
+    gp(lo)[dst] = gp(lo)[src];
   }
   // detectors sample the value rounded to float
   __device__ __forceinline__ float sample(int64_t i) const { return (float)get(i); }
@@ -181,7 +193,7 @@ struct MatPalette {
   __device__ __forceinline__ void get3(int64_t i, R m[3]) const {
     const int k = (int)ld_stream<NT>(id + i);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) m[c] = (R)tab[3 * k + c];
+    for (int c = 0; c < 3; ++c) m[c] = (R)gp(tab)[3 * k + c];
   }
 };
 
@@ -217,9 +229,9 @@ __device__ __forceinline__ void pml_face(const FaceDev& fd, const Dims& d, int i
   const int64_t snz = ax == 2 ? fd.t : d.nz;
   const int64_t li = ax == 0 ? l : i, lj = ax == 1 ? l : j, lk = ax == 2 ? l : k;
   const int64_t off = (li * sny + lj) * snz + lk;
-  T* __restrict__ psi = reinterpret_cast<T*>(IS_E ? fd.psiE : fd.psiH);
-  const double* bt = IS_E ? fd.bE : fd.bH;
-  const double* ct = IS_E ? fd.cE : fd.cH;
+  const gptr<T> psi = gp(reinterpret_cast<T*>(IS_E ? fd.psiE : fd.psiH));
+  const gptr<const double> bt = gp(IS_E ? fd.bE : fd.bH);
+  const gptr<const double> ct = gp(IS_E ? fd.cE : fd.cH);
   const T b = (T)bt[l], c = (T)ct[l];
   const T d0 = ax == 0 ? zx : (ax == 1 ? zy : yz);
   const T d1 = ax == 0 ? yx : (ax == 1 ? xy : xz);
@@ -267,12 +279,12 @@ __device__ __forceinline__ void objects_update_E(const ObjDev* __restrict__ objs
     const ObjDev& ob = objs[o];
     if (i < ob.x0 || i >= ob.x1 || j < ob.y0 || j >= ob.y1 || k < ob.z0 || k >= ob.z1) continue;
     const int64_t lo = ((int64_t)(i - ob.x0) * (ob.y1 - ob.y0) + (j - ob.y0)) * (ob.z1 - ob.z0) + (k - ob.z0);
-    const O* inv = reinterpret_cast<const O*>(ob.inv);
+    const gptr<const O> inv = gp(reinterpret_cast<const O*>(ob.inv));
     if (ob.kind == FDTD_OBJ_PLAIN) {
 #pragma unroll
       for (int c = 0; c < 3; ++c) e[c] = e[c] + (sc * (T)inv[lo * 3 + c]) * curl[c];
     } else if (ob.kind == FDTD_OBJ_ABSORBING) {
-      const O* fa = reinterpret_cast<const O*>(ob.absorb);
+      const gptr<const O> fa = gp(reinterpret_cast<const O*>(ob.absorb));
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const T f = (T)fa[lo * 3 + c];
@@ -280,7 +292,7 @@ __device__ __forceinline__ void objects_update_E(const ObjDev* __restrict__ objs
         e[c] = e[c] + ((sc * (T)inv[lo * 3 + c]) * curl[c]) / (T(1) + f);
       }
     } else {
-      const O* m = inv + lo * 9;
+      const gptr<const O> m = inv + lo * 9;
       T v[3];
 #pragma unroll
       // the association numpy's einsum uses for this 3-term contraction
@@ -348,8 +360,8 @@ struct MainArgs {
   const double* wave;  // row s = source s
   int64_t wave_cap;
   int64_t q;
-  // differentiation record (E kernel): the curl of this half step inside
-  // box `rb`, (3, cells of rb), when rec is not null
+  // differentiation record (E kernel, recording variant): the curl of this
+  // half step inside box `rb`, (3, cells of rb)
   R* rec;
   Box rb;
 };
@@ -359,13 +371,12 @@ struct MainArgs {
 // chunk of `cx` planes: the x-neighbours come from registers of the previous
 // plane, the z-neighbours from the neighbouring lane, the y-neighbours from
 // cache.
-template <class FS, class MS, typename O, bool OBJS, bool NT>
-__global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
+template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC = false>
+__device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile, int64_t q) {
   using T = typename FS::R;
   const Dims d = a.d;
   const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
   const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
-  const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
   const int lz = threadIdx.x & (bz - 1);
   const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
@@ -410,7 +421,7 @@ __global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
       curl[0] = (zero + df.zy) - df.yz;
       curl[1] = (zero + df.xz) - df.zx;
       curl[2] = (zero + df.yx) - df.xy;
-      if (a.rec) {
+      if (REC) {
         const int64_t r = box_index(a.rb, i, j, k);
         if (r >= 0) {
           const int64_t nr = box_cells(a.rb);
@@ -428,7 +439,7 @@ __global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
       if (OBJS) objects_update_E<T, O>(a.objs, a.nobjs, a.sc, i, j, k, curl, e);
       if (a.nfaces) pml_cell<true>(a.faces, a.nfaces, d, i, j, k, df, kc, e);
       for (int s = 0; s < a.npts; ++s)  // sources.py:109, registration order
-        if (a.pt_cell[s] == cell) e[2] = e[2] + (T)a.wave[s * a.wave_cap + a.q];
+        if (a.pt_cell[s] == cell) e[2] = e[2] + (T)gp(a.wave)[s * a.wave_cap + q];
 #pragma unroll
       for (int c = 0; c < 3; ++c) a.F.template put_s<NT>(c * cs + cell, e[c]);
     }
@@ -437,17 +448,21 @@ __global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
   }
 }
 
+template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC>
+__global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
+  main_E_tile<FS, MS, O, OBJS, NT, REC>(a, xcd_tile(blockIdx.x, gridDim.x), a.q);
+}
+
 // Main H kernel: the mirror image of the E kernel. Its differences are
 // forward ones, so workgroups take the tiles in decreasing order (the +y/+z
 // neighbour tiles were read just before, by the same XCD) and march along -x,
 // keeping E of plane i+1 in registers.
 template <class FS, class MS, bool NT>
-__global__ void __launch_bounds__(1024) k_main_H(MainArgs<FS, MS> a) {
+__device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
   using T = typename FS::R;
   const Dims d = a.d;
   const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
   const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
-  const unsigned tile = gridDim.x - 1 - xcd_tile(blockIdx.x, gridDim.x);
   const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
   const int lz = threadIdx.x & (bz - 1);
   const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
@@ -507,6 +522,65 @@ __global__ void __launch_bounds__(1024) k_main_H(MainArgs<FS, MS> a) {
     ez_next = ez;
     ey_next = ey;
   }
+}
+
+template <class FS, class MS, bool NT>
+__global__ void __launch_bounds__(1024) k_main_H(MainArgs<FS, MS> a) {
+  main_H_tile<FS, MS, NT>(a, gridDim.x - 1 - xcd_tile(blockIdx.x, gridDim.x));
+}
+
+// ------------------------------------------------------------ batched kernels
+// Many independent grids stepped by one launch per half step. The work list
+// is every grid's tiles in turn (first[s] = first item of grid s). Workgroup
+// w takes items w', w' + G, w' + 2G, ... (G workgroups, w' = w dealt to the
+// XCDs so that each XCD gets a contiguous run of items: neighbouring tiles
+// share halo planes in its L2). With G equal to the number of items every
+// workgroup has one tile, as for a single grid; a smaller G makes the grid of
+// workgroups persistent. The time index lives in device memory so that a
+// captured graph of steps can be replayed: the E kernel reads it for the
+// folded point sources, the H kernel advances it.
+template <class FS, class MS>
+struct BatchArgs {
+  const MainArgs<FS, MS>* sims;  // per grid, launch shape included
+  const int64_t* first;          // nsim + 1 entries
+  const int32_t* item_sim;       // grid of each work item
+  int64_t total;                 // work items
+  int64_t* q;                    // time index of the next E half step
+};
+
+template <class FS, class MS, typename O, bool OBJS, bool NT>
+__global__ void __launch_bounds__(1024) k_batch_E(BatchArgs<FS, MS> b) {
+  const int64_t q = *b.q;
+  for (int64_t it = xcd_tile(blockIdx.x, gridDim.x); it < b.total; it += gridDim.x) {
+    // the item and its grid are the same for the whole workgroup: say so,
+    // so that the grid's arguments are read with scalar loads (and, through
+    // the restrict reference of the tile body, not read again after the
+    // field stores)
+    const int64_t u = __builtin_amdgcn_readfirstlane((int)it);
+    const int s = __builtin_amdgcn_readfirstlane(gp(b.item_sim)[u]);
+    main_E_tile<FS, MS, O, OBJS, NT>(b.sims[s], (unsigned)(u - b.first[s]), q);
+  }
+}
+
+// H half step: the list walked from its end (the mirror of the E kernel's
+// order, as for one grid); one thread advances the time index (nothing else
+// in this kernel reads it, and the E kernel of this step has completed).
+template <class FS, class MS, bool NT>
+__global__ void __launch_bounds__(1024) k_batch_H(BatchArgs<FS, MS> b) {
+  for (int64_t it = xcd_tile(blockIdx.x, gridDim.x); it < b.total; it += gridDim.x) {
+    const int64_t u = __builtin_amdgcn_readfirstlane((int)(b.total - 1 - it));
+    const int s = __builtin_amdgcn_readfirstlane(gp(b.item_sim)[u]);
+    main_H_tile<FS, MS, NT>(b.sims[s], (unsigned)(u - b.first[s]));
+  }
+  // after the work: a branch at the top would keep the compiler from
+  // treating the tile loop's values as uniform
+  if (blockIdx.x == 0 && threadIdx.x == 0) b.q[0] = b.q[0] + 1;
+}
+
+// Sets the batched time index (a kernel rather than a memory copy, so that
+// nothing of the host stays referenced after the call).
+__global__ void k_set_time(int64_t* q, int64_t v) {
+  if (blockIdx.x == 0 && threadIdx.x == 0) q[0] = v;
 }
 
 // One CPML face applied on its own (faces registered after a periodic
@@ -610,14 +684,14 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
   for (int s = 0; s < a.nsrc; ++s) {
     const SrcDev& src = a.srcs[s];
     if (a.is_H && src.kind != FDTD_SRC_PLANE) continue;
-    const T w = (T)a.wave[s * a.wave_cap + a.q];
+    const T w = (T)gp(a.wave)[s * a.wave_cap + a.q];
     const int64_t base = (int64_t)(a.is_H ? src.compH : src.compE) * a.d.cs;
     for (int p = tid; p < src.n; p += blockDim.x) {
-      const int64_t cell = base + src.cells[p];
+      const int64_t cell = base + gp(src.cells)[p];
       if (src.kind == FDTD_SRC_POINT)
         a.F.put(cell, a.F.get(cell) + w);
       else if (src.kind == FDTD_SRC_LINE)
-        a.F.put(cell, a.F.get(cell) + (T)src.profile[p] * w);
+        a.F.put(cell, a.F.get(cell) + (T)gp(src.profile)[p] * w);
       else
         a.F.put(cell, (T)src.amplitude * w);
     }
@@ -633,10 +707,10 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
   for (int t = 0; t < a.ndet; ++t) {
     const DetDev& det = a.dets[t];
     const int64_t slot = a.counter - (a.is_H ? det.slot0H : det.slot0E);
-    Sample* buf = reinterpret_cast<Sample*>(a.is_H ? det.bufH : det.bufE) + slot * det.n * 3;
+    const gptr<Sample> buf = gp(reinterpret_cast<Sample*>(a.is_H ? det.bufH : det.bufE)) + slot * det.n * 3;
     for (int p = tid; p < det.n * 3; p += blockDim.x) {
       const int c = p % 3;
-      buf[p] = a.F.sample(c * a.d.cs + det.cells[p / 3]);
+      buf[p] = a.F.sample(c * a.d.cs + gp(det.cells)[p / 3]);
     }
   }
 }
@@ -790,18 +864,18 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
         const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
         const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
         const T sg0 = ax == (c0 + 1) % 3 ? T(-1) : T(1), sg1 = ax == (c1 + 1) % 3 ? T(-1) : T(1);
-        const T* pin = reinterpret_cast<const T*>(a.afaces[f].psiH[a.parity]);
-        T* pout = reinterpret_cast<T*>(a.afaces[f].psiH[1 - a.parity]);
+        const gptr<const T> pin = gp(reinterpret_cast<const T*>(a.afaces[f].psiH[a.parity]));
+        const gptr<T> pout = gp(reinterpret_cast<T*>(a.afaces[f].psiH[1 - a.parity]));
         const int l = face_l(fd, d, i, j, k);
         if (l >= 0) {
           const int64_t off = face_off(fd, d, i, j, k, l);
           const T w0 = pin[off] + sg0 * (c0 == 0 ? y0 : y1);
           const T w1 = pin[fd.slab + off] + sg1 * (c1 == 1 ? y1 : y2);
-          const T b = (T)fd.bH[l];
+          const T b = (T)gp(fd.bH)[l];
           pout[off] = b * w0;
           pout[fd.slab + off] = b * w1;
           if (l <= fd.t - 2) {
-            const T c = (T)fd.cH[l];
+            const T c = (T)gp(fd.cH)[l];
             add_at(e0, e1, e2, s0, -(c * w0));
             add_at(e0, e1, e2, s1, -(c * w1));
           }
@@ -817,7 +891,7 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
             const T yb1 = ax == 0 ? yz_prev : (ax == 1 ? yz_jm : yy_km);
             const T w0 = pin[off] + sg0 * yb0;
             const T w1 = pin[fd.slab + off] + sg1 * yb1;
-            const T c = (T)fd.cH[lb];
+            const T c = (T)gp(fd.cH)[lb];
             add_at(e0, e1, e2, s0, c * w0);
             add_at(e0, e1, e2, s1, c * w1);
           }
@@ -905,18 +979,18 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
         const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
         const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
         const T sg0 = ax == (c0 + 1) % 3 ? T(1) : T(-1), sg1 = ax == (c1 + 1) % 3 ? T(1) : T(-1);
-        const T* pin = reinterpret_cast<const T*>(a.afaces[f].psiE[a.parity]);
-        T* pout = reinterpret_cast<T*>(a.afaces[f].psiE[1 - a.parity]);
+        const gptr<const T> pin = gp(reinterpret_cast<const T*>(a.afaces[f].psiE[a.parity]));
+        const gptr<T> pout = gp(reinterpret_cast<T*>(a.afaces[f].psiE[1 - a.parity]));
         const int l = face_l(fd, d, i, j, k);
         if (l >= 0) {
           const int64_t off = face_off(fd, d, i, j, k, l);
           const T w0 = pin[off] + sg0 * KL(c0, cell);
           const T w1 = pin[fd.slab + off] + sg1 * KL(c1, cell);
-          const T b = (T)fd.bE[l];
+          const T b = (T)gp(fd.bE)[l];
           pout[off] = b * w0;
           pout[fd.slab + off] = b * w1;
           if (l >= 1) {
-            const T c = (T)fd.cE[l];
+            const T c = (T)gp(fd.cE)[l];
             add_at(h0, h1, h2, s0, c * w0);
             add_at(h0, h1, h2, s1, c * w1);
           }
@@ -930,7 +1004,7 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
             const int64_t off = face_off(fd, d, iu, ju, ku, lu);
             const T w0 = pin[off] + sg0 * KL(c0, cell + st);
             const T w1 = pin[fd.slab + off] + sg1 * KL(c1, cell + st);
-            const T c = (T)fd.cE[lu];
+            const T c = (T)gp(fd.cE)[lu];
             add_at(h0, h1, h2, s0, -(c * w0));
             add_at(h0, h1, h2, s1, -(c * w1));
           }

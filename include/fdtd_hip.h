@@ -131,6 +131,39 @@ int fdtd_half_step_H(FdtdPlan* plan, int64_t q, void* stream);
  * E half step then H half step at q, then q+1. */
 int fdtd_run(FdtdPlan* plan, int64_t q0, int64_t nsteps, void* stream);
 
+/* ---- Batches of grids ----------------------------------------------------
+ * No counterpart in the reference (one Grid steps at a time there,
+ * fdtd/grid.py:250-273): many independent grids, each a finalized plan with
+ * PML faces, objects and at most 4 point sources (no periodic boundaries, no
+ * detectors), all of one storage type, stepped together by one launch per
+ * half step. The grids' tiles form one work list; the time index lives on
+ * the device, so a captured graph of steps can be replayed. */
+typedef struct FdtdBatch FdtdBatch;
+
+/* Batch over n plans (borrowed: they must outlive the batch and stay
+ * unchanged). */
+int fdtd_batch_create(int n, FdtdPlan* const* plans, FdtdBatch** out);
+
+int fdtd_batch_destroy(FdtdBatch* batch);
+
+/* Launch shape: threads per workgroup (256, 512 or 1024), x planes per work
+ * item, and the number of workgroups (0, or at least the number of work
+ * items: one workgroup per item; fewer: a persistent grid of workgroups that
+ * walks the list). Discards a captured graph. */
+int fdtd_batch_set_launch(FdtdBatch* batch, int block_threads, int cx, int workgroups);
+
+/* Work items (tiles over all grids) of the current launch shape. */
+int fdtd_batch_work_items(const FdtdBatch* batch, int64_t* items);
+
+/* nsteps steps of every grid from time index q0, two launches per step. */
+int fdtd_batch_run(FdtdBatch* batch, int64_t q0, int64_t nsteps, void* stream);
+
+/* The same through a HIP graph of steps_per_graph steps, captured on the
+ * first call (and again when steps_per_graph changes) and replayed on a
+ * stream of the batch ordered after and before `stream`; the remainder of
+ * nsteps is launched directly. */
+int fdtd_batch_run_graph(FdtdBatch* batch, int64_t q0, int64_t nsteps, int64_t steps_per_graph, void* stream);
+
 /* ---- Differentiation (reverse mode) -------------------------------------
  * The reference differentiates through its torch backend (fdtd/backend.py:
  * 207-361 with autograd enabled): torch records every array operation of

@@ -1,0 +1,118 @@
+"""Batched stepping (fdtd_amd.batch.GridBatch) against the grids stepped one
+by one: the same kernels' arithmetic, so the results must be bit-identical,
+direct launches and captured-graph replays alike, for every storage type.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import fdtd_amd as fdtd
+from fdtd_amd.batch import GridBatch
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def make_grids(seed, n=5):
+    """n grids of uneven shapes with PML faces (some missing), objects of
+    every kind on some, 1-2 point sources and random initial fields."""
+    rs = np.random.RandomState(seed)
+    grids = []
+    for g in range(n):
+        shape = tuple(int(v) for v in rs.randint(10, 25, size=3))
+        grid = fdtd.Grid(shape, grid_spacing=50e-9, permittivity=rs.uniform(1.0, 4.0, size=shape + (1,)),
+                         courant_number=0.5)
+        t = int(rs.randint(2, 5))
+        for axis in range(3):
+            for high in (False, True):
+                if rs.rand() < 0.8:
+                    key = [slice(None)] * 3
+                    key[axis] = slice(-t, None) if high else slice(0, t)
+                    grid[tuple(key)] = fdtd.PML()
+        if g % 2 == 1:
+            grid[2:6, 3:7, 1:5] = fdtd.Object(permittivity=2.5)
+            grid[7:9, 1:4, 4:8] = fdtd.AbsorbingObject(permittivity=1.5, conductivity=1e5)
+        if g % 3 == 2:
+            grid[1:3, 5:8, 6:9] = fdtd.AnisotropicObject(permittivity=rs.uniform(1.5, 3.0, size=(2, 3, 3)))
+        for _ in range(int(rs.randint(1, 3))):
+            p = [int(rs.randint(0, s)) for s in shape]
+            grid[p[0], p[1], p[2]] = fdtd.PointSource(period=float(rs.uniform(8, 20)), pulse=bool(rs.rand() < 0.5))
+        grid.E = torch.from_numpy(rs.uniform(-1, 1, size=shape + (3,)))
+        grid.H = torch.from_numpy(rs.uniform(-1, 1, size=shape + (3,)))
+        grids.append(grid)
+    return grids
+
+
+def _fields(grids):
+    return [(g.full_precision("E").cpu().numpy(), g.full_precision("H").cpu().numpy()) for g in grids]
+
+
+@pytest.mark.parametrize("backend", ["hip.float64", "hip.float32", "hip.float32c"])
+def test_batch_matches_grids_stepped_alone(backend):
+    fdtd.set_backend(backend)
+    steps = 37
+    ref = make_grids(3)
+    for g in ref:
+        g.run(steps, progress_bar=False)
+    want = _fields(ref)
+    for graph, shape in ((False, (256, 1, 0)), (True, (512, 2, 64)), (True, (256, 3, 1000))):
+        grids = make_grids(3)
+        batch = GridBatch(grids, graph_steps=10)
+        batch.set_launch(*shape)
+        _say(f"[gpu] batch {backend} graph={graph} launch={shape}")
+        batch.run(steps, graph=graph)
+        got = _fields(grids)
+        for i, ((eg, hg), (ew, hw)) in enumerate(zip(got, want)):
+            assert np.array_equal(eg, ew) and np.array_equal(hg, hw), f"grid {i} differs ({backend}, {shape})"
+        assert all(g.time_steps_passed == steps for g in grids)
+    fdtd.set_backend("hip.float64")
+
+
+def test_batch_graph_replays_interleaved_with_eager_work():
+    """Replays of one captured graph, with host work and syncs between the
+    runs, against the grids stepped alone after each run."""
+    fdtd.set_backend("hip.float64")
+    ref = make_grids(7, n=4)
+    grids = make_grids(7, n=4)
+    batch = GridBatch(grids, graph_steps=8)
+    for rep in range(4):
+        for g in ref:
+            g.run(16, progress_bar=False)
+        batch.run(16, graph=True)
+        torch.cuda.synchronize()
+        _ = [float(g.E.abs().sum()) for g in grids]  # eager device work between replays
+        for (eg, hg), (ew, hw) in zip(_fields(grids), _fields(ref)):
+            assert np.array_equal(eg, ew) and np.array_equal(hg, hw), f"replay {rep}"
+
+
+def test_autotune_restores_state_and_keeps_results():
+    fdtd.set_backend("hip.float32")
+    ref = make_grids(11, n=3)
+    for g in ref:
+        g.run(12, progress_bar=False)
+    grids = make_grids(11, n=3)
+    batch = GridBatch(grids, graph_steps=4)
+    times = batch.autotune(steps=3, candidates=[(256, 1, 0), (512, 2, 0), (256, 4, 512)])
+    assert len(times) == 3 and batch.launch in times
+    assert all(g.time_steps_passed == 0 for g in grids)
+    batch.run(12)
+    for (eg, hg), (ew, hw) in zip(_fields(grids), _fields(ref)):
+        assert np.array_equal(eg, ew) and np.array_equal(hg, hw)
+    fdtd.set_backend("hip.float64")
+
+
+def test_batch_rejects_ineligible_grids():
+    fdtd.set_backend("hip.float64")
+    g1, g2 = make_grids(5, n=2)
+    g2[3, :, :] = fdtd.LineDetector(name="det")
+    with pytest.raises(ValueError):
+        GridBatch([g1, g2])
+    g3 = fdtd.Grid((12, 12, 12), grid_spacing=50e-9, courant_number=0.5)
+    g3[0, :, :] = fdtd.PeriodicBoundary()
+    g3[6, 6, 6] = fdtd.PointSource(period=10.0)
+    with pytest.raises(RuntimeError):
+        GridBatch([g1, g3]).run(2)
