@@ -364,6 +364,9 @@ int half_dispatch(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
 
 int adj_check_plan(const FdtdPlan* p) {
   if (int rc = check_plan(p)) return rc;
+  int side[3][2] = {{0, 0}, {0, 0}, {0, 0}};
+  for (const FaceHost& f : p->faces)
+    if (++side[f.axis][f.high ? 1 : 0] > 1) return fail(-1, "adjoint needs at most one PML face per side");
   if (p->dtype != FDTD_F64 && p->dtype != FDTD_F32) return fail(-1, "adjoint needs FDTD_F64 or FDTD_F32 storage");
   for (const Bnd& b : p->order)
     if (b.periodic) return fail(-1, "adjoint does not support periodic boundaries");
@@ -392,6 +395,8 @@ AdjArgs<T> adj_args(const FdtdAdjoint* a) {
   r.faces = p->d_faces;
   r.afaces = a->d_afaces;
   r.nfaces = (int32_t)p->faces.size();
+  for (int ax = 0; ax < 3; ++ax) r.face_of[ax][0] = r.face_of[ax][1] = -1;
+  for (size_t f = 0; f < p->faces.size(); ++f) r.face_of[p->faces[f].axis][p->faces[f].high ? 1 : 0] = (int32_t)f;
   r.db = a->db;
   r.pb = a->pb;
   r.grad = a->grad;

@@ -757,6 +757,7 @@ struct AdjArgs {
   const AdjFaceDev* afaces;
   int32_t nfaces;
   int32_t parity;
+  int32_t face_of[3][2];  // face index on the low / high side of each axis, or -1
   Box db;                // design box: grad += S lamE curl
   const T* curl;         // (3, cells of db) recorded for this step
   double* grad;          // (3, cells of db)
@@ -765,29 +766,117 @@ struct AdjArgs {
   int32_t bz_shift, cx, by;  // launch shape, as for the main kernels
 };
 
-// Slab index of cell (i,j,k) in face fd, or -1.
-__device__ __forceinline__ int face_l(const FaceDev& fd, const Dims& d, int i, int j, int k) {
-  const int ax = fd.axis;
-  const int g = ax == 0 ? i : (ax == 1 ? j : k);
-  const int n = ax == 0 ? d.nx : (ax == 1 ? d.ny : d.nz);
-  const int l = fd.high ? g - (n - fd.t) : g;
-  return (l >= 0 && l < fd.t) ? l : -1;
+// Slab layout constants of a face normal to AX: the slab is the grid with
+// axis AX shortened to t; `st` is the slab offset of one step along AX.
+template <int AX>
+__device__ __forceinline__ int64_t slab_off(const Dims& d, int t, int i, int j, int k, int l) {
+  if constexpr (AX == 0) return ((int64_t)l * d.ny + j) * d.nz + k;
+  else if constexpr (AX == 1) return ((int64_t)i * t + l) * d.nz + k;
+  else return ((int64_t)i * d.ny + j) * t + l;
+}
+template <int AX>
+__device__ __forceinline__ int64_t slab_step(const Dims& d, int t) {
+  if constexpr (AX == 0) return (int64_t)d.ny * d.nz;
+  else if constexpr (AX == 1) return d.nz;
+  else return 1;
 }
 
-// Offset of cell (i,j,k), slab index l, in the face's slab layout.
-__device__ __forceinline__ int64_t face_off(const FaceDev& fd, const Dims& d, int i, int j, int k, int l) {
-  const int ax = fd.axis;
-  const int64_t sny = ax == 1 ? fd.t : d.ny, snz = ax == 2 ? fd.t : d.nz;
-  const int64_t li = ax == 0 ? l : i, lj = ax == 1 ? l : j, lk = ax == 2 ? l : k;
-  return (li * sny + lj) * snz + lk;
+// The faces normal to AX at one cell of k_adj_H (see there): the face's own
+// psi transpose and the difference terms of this cell and of the face cell
+// one step back along AX (which holds the Y values yb0, yb1). The face's
+// axis is a compile-time constant, so components and signs are too.
+template <int AX, typename T>
+__device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int i, int j, int k, T yc0, T yc1, T yb0,
+                                            T yb1, T e[3]) {
+  constexpr int C0 = AX == 0 ? 1 : 0, C1 = AX == 2 ? 1 : 2;
+  constexpr int S0 = 3 - AX - C0, S1 = 3 - AX - C1;
+  // phi_H = psi_{C+1} - psi_{C+2} enters H' with a minus sign
+  const T sg0 = AX == (C0 + 1) % 3 ? T(-1) : T(1), sg1 = AX == (C1 + 1) % 3 ? T(-1) : T(1);
+  const Dims& d = a.d;
+  const int g = AX == 0 ? i : (AX == 1 ? j : k);
+  const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
+#pragma nounroll
+  for (int side = 0; side < 2; ++side) {
+    const int f = a.face_of[AX][side];
+    if (f < 0) continue;
+    const FaceDev& fd = a.faces[f];
+    const int t = fd.t;
+    const int l = side ? g - (n - t) : g;
+    if (l < 0 || l > t) continue;  // neither this cell nor the one back is in the slab
+    const gptr<const T> pin = gp(reinterpret_cast<const T*>(a.afaces[f].psiH[a.parity]));
+    const gptr<T> pout = gp(reinterpret_cast<T*>(a.afaces[f].psiH[1 - a.parity]));
+    const int64_t off = slab_off<AX>(d, t, i, j, k, l), slab = fd.slab;
+    if (l < t) {
+      const T w0 = pin[off] + sg0 * yc0;
+      const T w1 = pin[slab + off] + sg1 * yc1;
+      const T b = (T)gp(fd.bH)[l];
+      pout[off] = b * w0;
+      pout[slab + off] = b * w1;
+      if (l <= t - 2) {  // psi'(l) holds c(l) (E(l+1) - E(l))
+        const T c = (T)gp(fd.cH)[l];
+        e[S0] = e[S0] - c * w0;
+        e[S1] = e[S1] - c * w1;
+      }
+    }
+    if (l >= 1 && l - 1 <= t - 2) {  // ... and this cell is the E(l+1) of slab cell l-1
+      const int64_t ob = off - slab_step<AX>(d, t);
+      const T w0 = pin[ob] + sg0 * yb0;
+      const T w1 = pin[slab + ob] + sg1 * yb1;
+      const T c = (T)gp(fd.cH)[l - 1];
+      e[S0] = e[S0] + c * w0;
+      e[S1] = e[S1] + c * w1;
+    }
+  }
 }
 
-// v[s] += x for a runtime component index s, without indexing registers.
-template <typename T>
-__device__ __forceinline__ void add_at(T& v0, T& v1, T& v2, int s, T x) {
-  v0 = s == 0 ? v0 + x : v0;
-  v1 = s == 1 ? v1 + x : v1;
-  v2 = s == 2 ? v2 + x : v2;
+// The faces normal to AX at one cell of k_adj_E (see there): the face's own
+// psi transpose and the difference terms of this cell and of the face cell
+// one step ahead along AX.
+template <int AX, typename T>
+__device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int i, int j, int k, int64_t cell, T h[3]) {
+  constexpr int C0 = AX == 0 ? 1 : 0, C1 = AX == 2 ? 1 : 2;
+  constexpr int S0 = 3 - AX - C0, S1 = 3 - AX - C1;
+  // phi_E = psi_{C+1} - psi_{C+2} enters E' with a plus sign
+  const T sg0 = AX == (C0 + 1) % 3 ? T(1) : T(-1), sg1 = AX == (C1 + 1) % 3 ? T(1) : T(-1);
+  const Dims& d = a.d;
+  const int64_t cs = d.cs;
+  const int g = AX == 0 ? i : (AX == 1 ? j : k);
+  const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
+  const int64_t st = AX == 0 ? (int64_t)d.ny * d.nz : (AX == 1 ? (int64_t)d.nz : 1);
+  const T sc = a.sc;
+  auto KL = [&](int c, int64_t x) { return (sc * a.inv_eps[c * cs + x]) * a.lamE[c * cs + x]; };
+#pragma nounroll
+  for (int side = 0; side < 2; ++side) {
+    const int f = a.face_of[AX][side];
+    if (f < 0) continue;
+    const FaceDev& fd = a.faces[f];
+    const int t = fd.t;
+    const int l = side ? g - (n - t) : g;
+    if (l < -1 || l >= t) continue;  // neither this cell nor the one ahead is in the slab
+    const gptr<const T> pin = gp(reinterpret_cast<const T*>(a.afaces[f].psiE[a.parity]));
+    const gptr<T> pout = gp(reinterpret_cast<T*>(a.afaces[f].psiE[1 - a.parity]));
+    const int64_t off = slab_off<AX>(d, t, i, j, k, l), slab = fd.slab;
+    if (l >= 0) {
+      const T w0 = pin[off] + sg0 * KL(C0, cell);
+      const T w1 = pin[slab + off] + sg1 * KL(C1, cell);
+      const T b = (T)gp(fd.bE)[l];
+      pout[off] = b * w0;
+      pout[slab + off] = b * w1;
+      if (l >= 1) {  // psi'(l) holds c(l) (H(l) - H(l-1))
+        const T c = (T)gp(fd.cE)[l];
+        h[S0] = h[S0] + c * w0;
+        h[S1] = h[S1] + c * w1;
+      }
+    }
+    if (l + 1 >= 1 && l + 1 <= t - 1 && g + 1 < n) {  // ... and the H(l-1) of slab cell l+1
+      const int64_t ou = off + slab_step<AX>(d, t);
+      const T w0 = pin[ou] + sg0 * KL(C0, cell + st);
+      const T w1 = pin[slab + ou] + sg1 * KL(C1, cell + st);
+      const T c = (T)gp(fd.cE)[l + 1];
+      h[S0] = h[S0] - c * w0;
+      h[S1] = h[S1] - c * w1;
+    }
+  }
 }
 
 // Transpose of the H half step (grid.py:301-325, boundaries.py:421-431 and
@@ -858,44 +947,14 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
       e0 = e0 - t0;
       e1 = e1 - t1;
       e2 = e2 - t2;
-      for (int f = 0; f < a.nfaces; ++f) {
-        const FaceDev& fd = a.faces[f];
-        const int ax = fd.axis;
-        const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
-        const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
-        const T sg0 = ax == (c0 + 1) % 3 ? T(-1) : T(1), sg1 = ax == (c1 + 1) % 3 ? T(-1) : T(1);
-        const gptr<const T> pin = gp(reinterpret_cast<const T*>(a.afaces[f].psiH[a.parity]));
-        const gptr<T> pout = gp(reinterpret_cast<T*>(a.afaces[f].psiH[1 - a.parity]));
-        const int l = face_l(fd, d, i, j, k);
-        if (l >= 0) {
-          const int64_t off = face_off(fd, d, i, j, k, l);
-          const T w0 = pin[off] + sg0 * (c0 == 0 ? y0 : y1);
-          const T w1 = pin[fd.slab + off] + sg1 * (c1 == 1 ? y1 : y2);
-          const T b = (T)gp(fd.bH)[l];
-          pout[off] = b * w0;
-          pout[fd.slab + off] = b * w1;
-          if (l <= fd.t - 2) {
-            const T c = (T)gp(fd.cH)[l];
-            add_at(e0, e1, e2, s0, -(c * w0));
-            add_at(e0, e1, e2, s1, -(c * w1));
-          }
-        }
-        // this cell is the E(l+1) of the face cell one step back along ax,
-        // whose Y values this thread already holds
-        const int ib = ax == 0 ? i - 1 : i, jb = ax == 1 ? j - 1 : j, kb = ax == 2 ? k - 1 : k;
-        if (ib >= 0 && jb >= 0 && kb >= 0) {
-          const int lb = face_l(fd, d, ib, jb, kb);
-          if (lb >= 0 && lb <= fd.t - 2) {
-            const int64_t off = face_off(fd, d, ib, jb, kb, lb);
-            const T yb0 = ax == 0 ? yy_prev : (ax == 1 ? yx_jm : yx_km);
-            const T yb1 = ax == 0 ? yz_prev : (ax == 1 ? yz_jm : yy_km);
-            const T w0 = pin[off] + sg0 * yb0;
-            const T w1 = pin[fd.slab + off] + sg1 * yb1;
-            const T c = (T)gp(fd.cH)[lb];
-            add_at(e0, e1, e2, s0, c * w0);
-            add_at(e0, e1, e2, s1, c * w1);
-          }
-        }
+      {
+        T e[3] = {e0, e1, e2};
+        adj_faces_H<0>(a, i, j, k, y1, y2, yy_prev, yz_prev, e);
+        adj_faces_H<1>(a, i, j, k, y0, y2, yx_jm, yz_jm, e);
+        adj_faces_H<2>(a, i, j, k, y0, y1, yx_km, yy_km, e);
+        e0 = e[0];
+        e1 = e[1];
+        e2 = e[2];
       }
       const int64_t pp = box_index(a.pb, i, j, k);
       if (pp >= 0) {
@@ -942,7 +1001,6 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
   const T zero = T(0), sc = a.sc;
   auto Y = [&](int c, int64_t x) { return a.gcoef[c * cs + x] * a.lamE[c * cs + x]; };
-  auto KL = [&](int c, int64_t x) { return (sc * a.inv_eps[c * cs + x]) * a.lamE[c * cs + x]; };
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   T yz_next = zero, yy_next = zero;  // Y at plane i+1
   if (act && i1 < d.nx) {
@@ -973,42 +1031,14 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
       const T t1 = ((im ? y2 : zero) - (ip ? yz_next : zero)) - ((km ? y0 : zero) - (kp ? yx_kp : zero));
       const T t2 = ((jm ? y0 : zero) - (jp ? yx_jp : zero)) - ((im ? y1 : zero) - (ip ? yy_next : zero));
       T h0 = a.lamH[cell] + t0, h1 = a.lamH[cs + cell] + t1, h2 = a.lamH[2 * cs + cell] + t2;
-      for (int f = 0; f < a.nfaces; ++f) {
-        const FaceDev& fd = a.faces[f];
-        const int ax = fd.axis;
-        const int c0 = ax == 0 ? 1 : 0, c1 = ax == 2 ? 1 : 2;
-        const int s0 = 3 - ax - c0, s1 = 3 - ax - c1;
-        const T sg0 = ax == (c0 + 1) % 3 ? T(1) : T(-1), sg1 = ax == (c1 + 1) % 3 ? T(1) : T(-1);
-        const gptr<const T> pin = gp(reinterpret_cast<const T*>(a.afaces[f].psiE[a.parity]));
-        const gptr<T> pout = gp(reinterpret_cast<T*>(a.afaces[f].psiE[1 - a.parity]));
-        const int l = face_l(fd, d, i, j, k);
-        if (l >= 0) {
-          const int64_t off = face_off(fd, d, i, j, k, l);
-          const T w0 = pin[off] + sg0 * KL(c0, cell);
-          const T w1 = pin[fd.slab + off] + sg1 * KL(c1, cell);
-          const T b = (T)gp(fd.bE)[l];
-          pout[off] = b * w0;
-          pout[fd.slab + off] = b * w1;
-          if (l >= 1) {
-            const T c = (T)gp(fd.cE)[l];
-            add_at(h0, h1, h2, s0, c * w0);
-            add_at(h0, h1, h2, s1, c * w1);
-          }
-        }
-        // this cell is the H(l-1) of the face cell one step ahead along ax
-        const int iu = ax == 0 ? i + 1 : i, ju = ax == 1 ? j + 1 : j, ku = ax == 2 ? k + 1 : k;
-        if (iu < d.nx && ju < d.ny && ku < d.nz) {
-          const int lu = face_l(fd, d, iu, ju, ku);
-          if (lu >= 1) {
-            const int64_t st = ax == 0 ? sx : (ax == 1 ? sy : 1);
-            const int64_t off = face_off(fd, d, iu, ju, ku, lu);
-            const T w0 = pin[off] + sg0 * KL(c0, cell + st);
-            const T w1 = pin[fd.slab + off] + sg1 * KL(c1, cell + st);
-            const T c = (T)gp(fd.cE)[lu];
-            add_at(h0, h1, h2, s0, -(c * w0));
-            add_at(h0, h1, h2, s1, -(c * w1));
-          }
-        }
+      {
+        T h[3] = {h0, h1, h2};
+        adj_faces_E<0>(a, i, j, k, cell, h);
+        adj_faces_E<1>(a, i, j, k, cell, h);
+        adj_faces_E<2>(a, i, j, k, cell, h);
+        h0 = h[0];
+        h1 = h[1];
+        h2 = h[2];
       }
       a.lamH[cell] = h0;
       a.lamH[cs + cell] = h1;
