@@ -182,15 +182,15 @@ template <class FS>
 FS make_field(const FdtdPlan* p, bool H);
 template <>
 PlainField<double, double> make_field(const FdtdPlan* p, bool H) {
-  return {reinterpret_cast<double*>(H ? p->H : p->E)};
+  return {reinterpret_cast<double*>(H ? p->H : p->E), p->d.cs};
 }
 template <>
 PlainField<float, float> make_field(const FdtdPlan* p, bool H) {
-  return {reinterpret_cast<float*>(H ? p->H : p->E)};
+  return {reinterpret_cast<float*>(H ? p->H : p->E), p->d.cs};
 }
 template <>
 CompField make_field(const FdtdPlan* p, bool H) {
-  return {reinterpret_cast<float*>(H ? p->H : p->E), reinterpret_cast<uint8_t*>(H ? p->H_lo : p->E_lo)};
+  return {reinterpret_cast<float*>(H ? p->H : p->E), reinterpret_cast<uint32_t*>(H ? p->H_lo : p->E_lo), p->d.cs};
 }
 template <class MS>
 MS make_mat(const FdtdPlan* p, bool mu);

@@ -107,11 +107,27 @@ __device__ __forceinline__ void st_stream(T* p, T v) {
 }
 
 // ------------------------------------------------------------ field storage
+// Element i of a field is component i / cs of cell i % cs (component-major).
+// get/put/copy/sample take that element index; get_c/get3/put3 take a cell
+// and address its components (the main kernels' accesses).
 template <typename S, typename R_>
 struct PlainField {
   using R = R_;
   using Sample = S;
   S* p;
+  int64_t cs;
+  template <int C>
+  __device__ __forceinline__ R get_c(int64_t cell) const { return (R)gp(p)[C * cs + cell]; }
+  template <bool NT>
+  __device__ __forceinline__ void get3_s(int64_t cell, R v[3]) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = (R)ld_stream<NT>(p + c * cs + cell);
+  }
+  template <bool NT>
+  __device__ __forceinline__ void put3_s(int64_t cell, const R v[3]) const {
+#pragma unroll
+    for (int c = 0; c < 3; ++c) st_stream<NT>(p + c * cs + cell, (S)v[c]);
+  }
   __device__ __forceinline__ R get(int64_t i) const { return (R)gp(p)[i]; }
   template <bool NT>
   __device__ __forceinline__ R get_s(int64_t i) const { return (R)ld_stream<NT>(p + i); }
@@ -132,45 +148,68 @@ struct PlainField {
 // exponent, which is the correct rounding of the magnitude), then splits the
 // result. Relative error of a store: at most 2^-32. Values below the float
 // range (|v| < 2^-126) keep the float's precision only.
-__device__ __forceinline__ double comp_decode(float h, uint8_t q) {
+__device__ __forceinline__ double comp_decode(float h, uint32_t q) {
   const double w = (double)h;
-  const uint32_t lo32 = (uint32_t)__double2loint(w) | ((uint32_t)q << 21);
+  const uint32_t lo32 = (uint32_t)__double2loint(w) | (q << 21);
   return __hiloint2double(__double2hiint(w), (int)lo32);
 }
-__device__ __forceinline__ float comp_hi(double v, uint8_t& q) {
+__device__ __forceinline__ float comp_hi(double v, uint32_t& q) {
   const uint64_t u = (uint64_t)__double_as_longlong(v) + (1ull << 20);
-  q = (uint8_t)((u >> 21) & 0xffu);
+  q = (uint32_t)((u >> 21) & 0xffu);
   return (float)__longlong_as_double((long long)(u & ~((1ull << 29) - 1)));
 }
 
+// The floats are (3, N) like a plain field; the three compensation bytes of
+// a cell share one 32-bit word (byte c for component c), so a cell's
+// components cost one extra 4-byte access, not three byte accesses.
 struct CompField {
   using R = double;
   using Sample = float;
-  float* hi;
-  uint8_t* lo;
-  __device__ __forceinline__ double get(int64_t i) const { return comp_decode(gp(hi)[i], gp(lo)[i]); }
-  template <bool NT>
-  __device__ __forceinline__ double get_s(int64_t i) const {
-    return comp_decode(ld_stream<NT>(hi + i), ld_stream<NT>(lo + i));
-  }
-  template <bool NT>
-  __device__ __forceinline__ void put_s(int64_t i, double v) const {
-    uint8_t q;
-    const float h = comp_hi(v, q);
-    st_stream<NT>(hi + i, h);
-    st_stream<NT>(lo + i, q);
+  float* hi;     // (3, N)
+  uint32_t* lo;  // (N)
+  int64_t cs;
+  __device__ __forceinline__ int comp_of(int64_t i) const { return i >= 2 * cs ? 2 : (i >= cs ? 1 : 0); }
+  __device__ __forceinline__ double get(int64_t i) const {
+    const int c = comp_of(i);
+    return comp_decode(gp(hi)[i], (gp(lo)[i - c * cs] >> (8 * c)) & 0xffu);
   }
   __device__ __forceinline__ void put(int64_t i, double v) const {
-    uint8_t q;
+    const int c = comp_of(i);
+    uint32_t q;
     gp(hi)[i] = comp_hi(v, q);
-    gp(lo)[i] = q;
+    const gptr<uint32_t> w = gp(lo) + (i - c * cs);
+    *w = (*w & ~(0xffu << (8 * c))) | (q << (8 * c));
   }
   __device__ __forceinline__ void copy(int64_t dst, int64_t src) const {
+    const int c = comp_of(dst);
     gp(hi)[dst] = gp(hi)[src];
-    gp(lo)[dst] = gp(lo)[src];
+    const uint32_t q = (gp(lo)[src - comp_of(src) * cs] >> (8 * comp_of(src))) & 0xffu;
+    const gptr<uint32_t> w = gp(lo) + (dst - c * cs);
+    *w = (*w & ~(0xffu << (8 * c))) | (q << (8 * c));
   }
   // detectors sample the value rounded to float
   __device__ __forceinline__ float sample(int64_t i) const { return (float)get(i); }
+  template <int C>
+  __device__ __forceinline__ double get_c(int64_t cell) const {
+    return comp_decode(gp(hi)[C * cs + cell], (gp(lo)[cell] >> (8 * C)) & 0xffu);
+  }
+  template <bool NT>
+  __device__ __forceinline__ void get3_s(int64_t cell, double v[3]) const {
+    const uint32_t w = ld_stream<NT>(lo + cell);
+#pragma unroll
+    for (int c = 0; c < 3; ++c) v[c] = comp_decode(ld_stream<NT>(hi + c * cs + cell), (w >> (8 * c)) & 0xffu);
+  }
+  template <bool NT>
+  __device__ __forceinline__ void put3_s(int64_t cell, const double v[3]) const {
+    uint32_t w = 0;
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      uint32_t q;
+      st_stream<NT>(hi + c * cs + cell, comp_hi(v[c], q));
+      w |= q << (8 * c);
+    }
+    st_stream<NT>(lo + cell, w);
+  }
 };
 
 // --------------------------------------------------------------- materials
@@ -387,27 +426,29 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
   int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
   T hz_prev = zero, hy_prev = zero;
   if (act && i0 > 0) {
-    hz_prev = a.G.get(2 * cs + cell - sx);
-    hy_prev = a.G.get(cs + cell - sx);
+    hz_prev = a.G.template get_c<2>(cell - sx);
+    hy_prev = a.G.template get_c<1>(cell - sx);
   }
   for (int i = i0; i < i1; ++i, cell += sx) {
     T hx = zero, hy = zero, hz = zero;
     if (act) {
-      hx = a.G.get(cell);
-      hy = a.G.get(cs + cell);
-      hz = a.G.get(2 * cs + cell);
+      T hv[3];
+      a.G.template get3_s<false>(cell, hv);
+      hx = hv[0];
+      hy = hv[1];
+      hz = hv[2];
     }
     T hy_km = __shfl_up(hy, 1);
     T hx_km = __shfl_up(hx, 1);
     if (act) {
       if (lz == 0 && k > 0) {
-        hy_km = a.G.get(cs + cell - 1);
-        hx_km = a.G.get(cell - 1);
+        hy_km = a.G.template get_c<1>(cell - 1);
+        hx_km = a.G.template get_c<0>(cell - 1);
       }
       T hz_jm = zero, hx_jm = zero;
       if (j > 0) {
-        hz_jm = a.G.get(2 * cs + cell - sy);
-        hx_jm = a.G.get(cell - sy);
+        hz_jm = a.G.template get_c<2>(cell - sy);
+        hx_jm = a.G.template get_c<0>(cell - sy);
       }
       // backward differences; an absent one contributes nothing (grid.py:67-74)
       Diffs<T> df;
@@ -431,17 +472,17 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
       }
       T m[3], kc[3], e[3];
       a.M.template get3<NT>(cell, m);
+      a.F.template get3_s<NT>(cell, e);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         kc[c] = a.sc * m[c];
-        e[c] = a.F.template get_s<NT>(c * cs + cell) + kc[c] * curl[c];
+        e[c] = e[c] + kc[c] * curl[c];
       }
       if (OBJS) objects_update_E<T, O>(a.objs, a.nobjs, a.sc, i, j, k, curl, e);
       if (a.nfaces) pml_cell<true>(a.faces, a.nfaces, d, i, j, k, df, kc, e);
       for (int s = 0; s < a.npts; ++s)  // sources.py:109, registration order
         if (a.pt_cell[s] == cell) e[2] = e[2] + (T)gp(a.wave)[s * a.wave_cap + q];
-#pragma unroll
-      for (int c = 0; c < 3; ++c) a.F.template put_s<NT>(c * cs + cell, e[c]);
+      a.F.template put3_s<NT>(cell, e);
     }
     hz_prev = hz;
     hy_prev = hy;
@@ -473,27 +514,29 @@ __device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   T ez_next = zero, ey_next = zero;  // E at plane i+1
   if (act && i1 < d.nx) {
-    ez_next = a.G.get(2 * cs + cell + sx);
-    ey_next = a.G.get(cs + cell + sx);
+    ez_next = a.G.template get_c<2>(cell + sx);
+    ey_next = a.G.template get_c<1>(cell + sx);
   }
   for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
     T ex = zero, ey = zero, ez = zero;
     if (act) {
-      ex = a.G.get(cell);
-      ey = a.G.get(cs + cell);
-      ez = a.G.get(2 * cs + cell);
+      T ev[3];
+      a.G.template get3_s<false>(cell, ev);
+      ex = ev[0];
+      ey = ev[1];
+      ez = ev[2];
     }
     T ey_kp = __shfl_down(ey, 1);
     T ex_kp = __shfl_down(ex, 1);
     if (act) {
       if (lz == bz - 1 && k < d.nz - 1) {
-        ey_kp = a.G.get(cs + cell + 1);
-        ex_kp = a.G.get(cell + 1);
+        ey_kp = a.G.template get_c<1>(cell + 1);
+        ex_kp = a.G.template get_c<0>(cell + 1);
       }
       T ez_jp = zero, ex_jp = zero;
       if (j < d.ny - 1) {
-        ez_jp = a.G.get(2 * cs + cell + sy);
-        ex_jp = a.G.get(cell + sy);
+        ez_jp = a.G.template get_c<2>(cell + sy);
+        ex_jp = a.G.template get_c<0>(cell + sy);
       }
       // forward differences; an absent one contributes nothing (grid.py:42-49)
       Diffs<T> df;
@@ -509,15 +552,15 @@ __device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__
       curl[2] = (zero + df.yx) - df.xy;
       T m[3], kc[3], h[3];
       a.M.template get3<NT>(cell, m);
+      a.F.template get3_s<NT>(cell, h);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         kc[c] = a.sc * m[c];
-        h[c] = a.F.template get_s<NT>(c * cs + cell) - kc[c] * curl[c];
+        h[c] = h[c] - kc[c] * curl[c];
       }
       // objects leave H unchanged (objects.py:131-137, 223-229, 271-277)
       if (a.nfaces) pml_cell<false>(a.faces, a.nfaces, d, i, j, k, df, kc, h);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) a.F.template put_s<NT>(c * cs + cell, h[c]);
+      a.F.template put3_s<NT>(cell, h);
     }
     ez_next = ez;
     ey_next = ey;

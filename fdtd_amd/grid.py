@@ -96,8 +96,9 @@ class Grid:
             # grid.E/H[...] changes the float part only: the byte left there
             # lies below the new float's last bit, so the written value is
             # kept to float32 precision and every other cell keeps 32 bits.
-            self._E_lo = torch.zeros(soa, dtype=torch.uint8, device=dev)
-            self._H_lo = torch.zeros(soa, dtype=torch.uint8, device=dev)
+            # one 32-bit word per cell: byte c holds component c's bits
+            self._E_lo = torch.zeros(soa[1:], dtype=torch.int32, device=dev)
+            self._H_lo = torch.zeros(soa[1:], dtype=torch.int32, device=dev)
         # materials read through a <=256-entry float64 palette: "auto" uses one
         # in the compensated mode when the grid has few distinct materials
         self.material_palette = "auto"
@@ -150,8 +151,9 @@ class Grid:
             # and the next 8 mantissa bits (rounded at bit 21 of the double)
             v = value.to(device=dst.device, dtype=torch.float64).expand_as(view)
             bits = v.contiguous().view(torch.int64) + (1 << 20)
-            lo = _soa_view(self._E_lo if dst is self._E else self._H_lo)
-            lo.copy_(((bits >> 21) & 0xFF).to(torch.uint8))
+            q = (bits >> 21) & 0xFF  # (Nx, Ny, Nz, 3)
+            word = q[..., 0] | (q[..., 1] << 8) | (q[..., 2] << 16)
+            (self._E_lo if dst is self._E else self._H_lo).copy_(word.to(torch.int32))
             view.copy_((bits & ~((1 << 29) - 1)).view(torch.float64).to(torch.float32))
             return
         view.copy_(value.to(device=dst.device, dtype=dst.dtype).expand_as(view))
@@ -176,8 +178,9 @@ class Grid:
         soa = self._E if field == "E" else self._H
         if not self.compensated:
             return _soa_view(soa.to(torch.float64))
-        lo = self._E_lo if field == "E" else self._H_lo
-        bits = soa.to(torch.float64).view(torch.int64) | (lo.to(torch.int64) << 21)
+        word = (self._E_lo if field == "E" else self._H_lo).to(torch.int64)
+        q = torch.stack([(word >> (8 * c)) & 0xFF for c in range(3)])  # (3, Nx, Ny, Nz)
+        bits = soa.to(torch.float64).view(torch.int64) | (q << 21)
         return _soa_view(bits.view(torch.float64))
 
     @H.setter

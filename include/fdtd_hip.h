@@ -56,10 +56,11 @@ int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number,
 
 int fdtd_plan_destroy(FdtdPlan* plan);
 
-/* FDTD_F32C only, required: the 8-bit compensation terms of E and H, uint8
- * (3, nx, ny, nz) each. A value is the float E (its top 24 significant bits)
- * extended by the 8 mantissa bits that follow, held in the byte: 32
- * significant bits in all; a zero byte leaves the float's value unchanged. */
+/* FDTD_F32C only, required: the compensation terms of E and H, one uint32
+ * per cell each, (nx, ny, nz); byte c of a cell's word belongs to component
+ * c. A value is the float E (its top 24 significant bits) extended by the 8
+ * mantissa bits that follow, held in its byte: 32 significant bits in all; a
+ * zero byte leaves the float's value unchanged. */
 int fdtd_plan_set_compensation(FdtdPlan* plan, void* E_lo, void* H_lo);
 
 /* Optional, any dtype: read the grid materials through a palette instead of
