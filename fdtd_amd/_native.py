@@ -37,6 +37,10 @@ _SIGNATURES = {
     "fdtd_plan_add_detector": ([_vp, _c_int, _pi64, _vp, _vp, _c_i64, _c_i64, _c_i64], _c_int),
     "fdtd_plan_finalize": ([_vp], _c_int),
     "fdtd_plan_detector_counts": ([_vp, _c_int, _pi64, _pi64], _c_int),
+    "fdtd_plan_add_current_detector": ([_vp, _c_int, _pi32, _pi32, _pi32, _c_dbl, _vp, _c_i64, _c_i64], _c_int),
+    "fdtd_plan_add_soft_source": ([_vp, _c_i64, _c_dbl, _c_dbl, _c_int, _vp, _vp, _c_i64, _c_i64], _c_int),
+    "fdtd_plan_current_detector_count": ([_vp, _c_int, _pi64], _c_int),
+    "fdtd_plan_soft_source_count": ([_vp, _c_int, _pi64], _c_int),
     "fdtd_half_step_E": ([_vp, _c_i64, _vp], _c_int),
     "fdtd_half_step_H": ([_vp, _c_i64, _vp], _c_int),
     "fdtd_run": ([_vp, _c_i64, _c_i64, _vp], _c_int),
@@ -175,6 +179,18 @@ class Plan:
         check(self._L.fdtd_plan_add_detector(self._h, len(cells), _i64_array(cells), bufE.data_ptr(),
                                              bufH.data_ptr(), bufE.shape[0], count_E, count_H))
         self._keep += [bufE, bufH]
+
+    def add_current_detector(self, px, py, pz, spacing, buf, count):
+        n = len(px)
+        arr = lambda v: (ctypes.c_int32 * n)(*[int(a) for a in v])
+        check(self._L.fdtd_plan_add_current_detector(self._h, n, arr(px), arr(py), arr(pz), float(spacing),
+                                                     buf.data_ptr(), buf.shape[0], count))
+        self._keep.append(buf)
+
+    def add_soft_source(self, cell, impedance, spacing, detector, vin, vout, count):
+        check(self._L.fdtd_plan_add_soft_source(self._h, int(cell), float(impedance), float(spacing), int(detector),
+                                                vin.data_ptr(), vout.data_ptr(), vin.shape[0], count))
+        self._keep += [vin, vout]
 
     def finalize(self):
         check(self._L.fdtd_plan_finalize(self._h))

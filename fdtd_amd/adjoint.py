@@ -84,6 +84,10 @@ class IntensityGradient:
                 raise NotImplementedError("gradients do not support anisotropic objects")
             if obj is not design and _overlap(obj, design):
                 raise ValueError("another object overlaps the design region")
+        for src in grid.sources:
+            if type(src).__name__ == "SoftArbitraryPointSource" and src.impedance > 0:
+                raise NotImplementedError("a resistive soft source feeds the fields back into the source: "
+                                          "its transpose is not implemented")
         if checkpoint not in ("sqrt", "record", "auto"):
             raise ValueError("checkpoint must be 'sqrt', 'record' or 'auto'")
         self.grid = grid

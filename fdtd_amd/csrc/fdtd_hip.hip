@@ -48,6 +48,18 @@ struct SrcHost {
   std::vector<int64_t> cells;
   std::vector<double> profile;
   double amplitude;
+  // FDTD_SRC_SOFT
+  double spacing = 0;
+  int det = -1;
+  void* vin = nullptr;
+  void* vout = nullptr;
+  int64_t cap = 0, count = 0;
+};
+struct CurHost {
+  std::vector<int32_t> pts;  // px[n], py[n], pz[n]
+  double spacing;
+  void* buf;
+  int64_t cap, count;
 };
 struct DetHost {
   std::vector<int64_t> cells;
@@ -77,6 +89,7 @@ struct FdtdPlan {
   std::vector<ObjHost> objs;
   std::vector<SrcHost> srcs;
   std::vector<DetHost> dets;
+  std::vector<CurHost> curs;
   std::vector<Bnd> order;
   const double* wave = nullptr;
   int64_t wave_cap = 0;
@@ -87,6 +100,7 @@ struct FdtdPlan {
   ObjDev* d_objs = nullptr;
   SrcDev* d_srcs = nullptr;
   DetDev* d_dets = nullptr;
+  CurDev* d_curs = nullptr;
   std::vector<FaceDev> h_faces;
   int n_fused = 0;           // leading faces folded into the main kernels
   std::vector<Bnd> stages;   // boundaries applied after the main kernel
@@ -168,6 +182,7 @@ void free_tables(FdtdPlan* p) {
   p->d_objs = nullptr;
   p->d_srcs = nullptr;
   p->d_dets = nullptr;
+  p->d_curs = nullptr;
 }
 
 int select_device(const FdtdPlan* p) {
@@ -298,7 +313,7 @@ int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
   if (!p->fold_points)
     for (const SrcHost& s : p->srcs) any_src |= (!is_H || s.kind == FDTD_SRC_PLANE);
   const bool rec_plane = !is_H && p->rec_plane;
-  if (any_src || !p->dets.empty() || rec_plane) {
+  if (any_src || !p->dets.empty() || rec_plane || (is_H && !p->curs.empty())) {
     TailArgs<FS> t{};
     t.d = p->d;
     t.F = a.F;
@@ -315,15 +330,21 @@ int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
     t.wave_cap = p->wave_cap;
     t.q = q;
     t.counter = is_H ? p->nH : p->nE;
+    t.counter_H = p->nH;
+    t.ncur = (int32_t)p->curs.size();
+    t.curs = p->d_curs;
     hipLaunchKernelGGL(k_tail<FS>, dim3(1), dim3(256), 0, st, t);
   }
   FDTD_HIP(hipGetLastError());
   if (is_H) {
     ++p->nH;
     for (DetHost& dh : p->dets) ++dh.countH;
+    for (CurHost& ch : p->curs) ++ch.count;
   } else {
     ++p->nE;
     for (DetHost& dh : p->dets) ++dh.countE;
+    for (SrcHost& sh : p->srcs)
+      if (sh.kind == FDTD_SRC_SOFT) ++sh.count;
   }
   return 0;
 }
@@ -354,6 +375,16 @@ int check_step_room(const FdtdPlan* p, int64_t q0, int64_t nsteps, bool is_H_onl
     const int64_t needH = dh.countH + (e_only ? 0 : nsteps);
     if (needE > dh.cap || needH > dh.cap)
       return fail(-4, "detector " + std::to_string(i) + " buffer too small");
+  }
+  for (size_t i = 0; i < p->curs.size(); ++i)
+    if (p->curs[i].count + (e_only ? 0 : nsteps) > p->curs[i].cap)
+      return fail(-4, "current detector " + std::to_string(i) + " buffer too small");
+  for (const SrcHost& sh : p->srcs) {
+    if (sh.kind != FDTD_SRC_SOFT) continue;
+    if (sh.count + (is_H_only ? 0 : nsteps) > sh.cap) return fail(-4, "soft source voltage buffer too small");
+    // the reference reads its detector's last current for q > 0 (sources.py:613-616)
+    if (!is_H_only && nsteps > 0 && q0 > 0 && p->curs[sh.det].count == 0)
+      return fail(-5, "soft source at time index > 0 before its current detector has a sample");
   }
   return 0;
 }
@@ -701,6 +732,71 @@ int fdtd_plan_add_source(FdtdPlan* p, int kind, int n, const int64_t* cells, con
   return 0;
 }
 
+int fdtd_plan_add_current_detector(FdtdPlan* p, int n, const int32_t* px, const int32_t* py, const int32_t* pz,
+                                   double spacing, void* bufI, int64_t capacity, int64_t count) {
+  if (!p) return fail(-1, "null plan");
+  if (n < 1 || !px || !py || !pz || !bufI) return fail(-1, "current detector without points or buffer");
+  if (count < 0 || count > capacity) return fail(-1, "bad current detector count");
+  CurHost c;
+  c.pts.resize(3 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    // numpy indexing: -n <= index < n, and the loop reads index - 1
+    if (px[i] < 0 || px[i] >= p->d.nx || py[i] < 0 || py[i] >= p->d.ny || pz[i] < 0 || pz[i] >= p->d.nz)
+      return fail(-1, "current detector point outside the grid");
+    c.pts[i] = px[i];
+    c.pts[n + i] = py[i];
+    c.pts[2 * n + i] = pz[i];
+  }
+  c.spacing = spacing;
+  c.buf = bufI;
+  c.cap = capacity;
+  c.count = count;
+  p->curs.push_back(std::move(c));
+  p->finalized = false;
+  return 0;
+}
+
+int fdtd_plan_add_soft_source(FdtdPlan* p, int64_t cell, double impedance, double spacing, int detector, void* vin,
+                              void* vout, int64_t capacity, int64_t count) {
+  if (!p) return fail(-1, "null plan");
+  if (!cell_ok(p, cell)) return fail(-1, "source cell outside the grid");
+  if (!vin || !vout || count < 0 || count > capacity) return fail(-1, "bad soft source buffers");
+  if (detector < 0) return fail(-1, "bad current detector index");
+  SrcHost s;
+  s.kind = FDTD_SRC_SOFT;
+  s.compE = 2;
+  s.compH = 0;
+  s.cells.assign(1, cell);
+  s.amplitude = impedance;
+  s.spacing = spacing;
+  s.det = detector;
+  s.vin = vin;
+  s.vout = vout;
+  s.cap = capacity;
+  s.count = count;
+  p->srcs.push_back(std::move(s));
+  p->finalized = false;
+  return 0;
+}
+
+int fdtd_plan_current_detector_count(const FdtdPlan* p, int index, int64_t* count) {
+  if (!p || !count) return fail(-1, "null argument");
+  if (index < 0 || index >= (int)p->curs.size()) return fail(-1, "current detector index out of range");
+  *count = p->curs[index].count;
+  return 0;
+}
+
+int fdtd_plan_soft_source_count(const FdtdPlan* p, int index, int64_t* count) {
+  if (!p || !count) return fail(-1, "null argument");
+  int k = 0;
+  for (const SrcHost& sh : p->srcs)
+    if (sh.kind == FDTD_SRC_SOFT && k++ == index) {
+      *count = sh.count;
+      return 0;
+    }
+  return fail(-1, "soft source index out of range");
+}
+
 int fdtd_plan_set_waveforms(FdtdPlan* p, const double* table, int64_t capacity) {
   if (!p) return fail(-1, "null plan");
   if (capacity < 0 || (capacity > 0 && !table)) return fail(-1, "bad waveform table");
@@ -763,11 +859,15 @@ int fdtd_plan_finalize(FdtdPlan* p) {
   }
   // point sources fold into the E kernel when nothing runs between the main
   // kernel and the sources (no periodic or staged face) and no detector samples
-  p->fold_points = !p->srcs.empty() && (int)p->srcs.size() <= FOLDED_POINTS && p->dets.empty();
+  p->fold_points = !p->srcs.empty() && (int)p->srcs.size() <= FOLDED_POINTS && p->dets.empty() && p->curs.empty();
   for (const SrcHost& s : p->srcs) p->fold_points = p->fold_points && s.kind == FDTD_SRC_POINT;
   for (const Bnd& b : p->order) p->fold_points = p->fold_points && !b.periodic;
+  for (const SrcHost& sh : p->srcs)
+    if (sh.kind == FDTD_SRC_SOFT && (sh.det < 0 || sh.det >= (int)p->curs.size()))
+      return fail(-1, "soft source refers to a missing current detector");
   // byte layout of the table block
-  const size_t nf = p->faces.size(), no = p->objs.size(), ns = p->srcs.size(), nd = p->dets.size();
+  const size_t nf = p->faces.size(), no = p->objs.size(), ns = p->srcs.size(), nd = p->dets.size(),
+               nc = p->curs.size();
   size_t off = 0;
   const size_t o_faces = off;
   off = align_up(off + nf * sizeof(FaceDev));
@@ -777,6 +877,13 @@ int fdtd_plan_finalize(FdtdPlan* p) {
   off = align_up(off + ns * sizeof(SrcDev));
   const size_t o_dets = off;
   off = align_up(off + nd * sizeof(DetDev));
+  const size_t o_curs = off;
+  off = align_up(off + nc * sizeof(CurDev));
+  std::vector<size_t> o_cpts(nc);
+  for (size_t i = 0; i < nc; ++i) {
+    o_cpts[i] = off;
+    off = align_up(off + p->curs[i].pts.size() * sizeof(int32_t));
+  }
   std::vector<size_t> o_coef(nf), o_scell(ns), o_sprof(ns), o_dcell(nd);
   for (size_t i = 0; i < nf; ++i) {
     o_coef[i] = off;
@@ -843,6 +950,11 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     sd.cells = reinterpret_cast<const int64_t*>(base + o_scell[i]);
     sd.profile = s.profile.empty() ? nullptr : reinterpret_cast<const double*>(base + o_sprof[i]);
     sd.amplitude = s.amplitude;
+    sd.spacing = s.spacing;
+    sd.det = s.det;
+    sd.vin = s.vin;
+    sd.vout = s.vout;
+    sd.slot0 = p->nE - s.count;
     std::memcpy(host.data() + o_srcs + i * sizeof(SrcDev), &sd, sizeof(SrcDev));
   }
   for (size_t i = 0; i < nd; ++i) {
@@ -857,11 +969,23 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     dd.slot0H = p->nH - dh.countH;
     std::memcpy(host.data() + o_dets + i * sizeof(DetDev), &dd, sizeof(DetDev));
   }
+  for (size_t i = 0; i < nc; ++i) {
+    const CurHost& ch = p->curs[i];
+    std::memcpy(host.data() + o_cpts[i], ch.pts.data(), ch.pts.size() * sizeof(int32_t));
+    CurDev cd{};
+    cd.n = (int32_t)(ch.pts.size() / 3);
+    cd.pts = reinterpret_cast<const int32_t*>(base + o_cpts[i]);
+    cd.spacing = ch.spacing;
+    cd.buf = ch.buf;
+    cd.slot0 = p->nH - ch.count;
+    std::memcpy(host.data() + o_curs + i * sizeof(CurDev), &cd, sizeof(CurDev));
+  }
   FDTD_HIP(hipMemcpy(p->dev, host.data(), total, hipMemcpyHostToDevice));
   p->d_faces = reinterpret_cast<FaceDev*>(base + o_faces);
   p->d_objs = reinterpret_cast<ObjDev*>(base + o_objs);
   p->d_srcs = reinterpret_cast<SrcDev*>(base + o_srcs);
   p->d_dets = reinterpret_cast<DetDev*>(base + o_dets);
+  p->d_curs = reinterpret_cast<CurDev*>(base + o_curs);
   p->finalized = true;
   return 0;
 }

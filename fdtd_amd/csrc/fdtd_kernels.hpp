@@ -63,7 +63,22 @@ struct SrcDev {
   int32_t kind, n, compE, compH;
   const int64_t* cells;
   const double* profile;
-  double amplitude;
+  double amplitude;  // FDTD_SRC_SOFT: the impedance
+  // FDTD_SRC_SOFT only
+  double spacing;
+  int32_t det, pad;  // current detector read
+  void* vin;
+  void* vout;
+  int64_t slot0;     // slot written at E half-step counter c is c - slot0
+};
+
+// Current detector: n points (px, py, pz), loop currents after each H half step.
+struct CurDev {
+  int32_t n, pad;
+  const int32_t* pts;  // px[n], py[n], pz[n]
+  double spacing;
+  void* buf;           // (capacity, n)
+  int64_t slot0;       // slot written at H half-step counter c is c - slot0
 };
 
 struct DetDev {
@@ -710,6 +725,9 @@ struct TailArgs {
   int64_t wave_cap;
   int64_t q;
   int64_t counter;  // half steps of this kind already taken by the plan
+  int64_t counter_H;  // H half steps taken (the soft sources' current sample)
+  int32_t ncur;
+  const CurDev* curs;
   // differentiation record (E half step): the field inside box `rb` after
   // the sources, (3, cells of rb), when rec is not null
   typename FS::R* rec;
@@ -729,6 +747,23 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
     if (a.is_H && src.kind != FDTD_SRC_PLANE) continue;
     const T w = (T)gp(a.wave)[s * a.wave_cap + a.q];
     const int64_t base = (int64_t)(a.is_H ? src.compH : src.compE) * a.d.cs;
+    if (src.kind == FDTD_SRC_SOFT) {
+      // sources.py:604-631: the voltage v of the waveform, plus R times the
+      // current its detector measured after the last H half step
+      if (tid == 0) {
+        const CurDev& cd = a.curs[src.det];
+        const T cur = a.q > 0 ? (T)gp(reinterpret_cast<const Sample*>(cd.buf))[(a.counter_H - 1 - cd.slot0) * cd.n]
+                              : T(0);
+        const T u = src.amplitude > 0 ? w + (T)src.amplitude * cur : w;
+        const int64_t cell = base + gp(src.cells)[0];
+        a.F.put(cell, a.F.get(cell) + u / (T)src.spacing);
+        const int64_t slot = a.counter - src.slot0;
+        gp(reinterpret_cast<Sample*>(src.vin))[slot] = (Sample)w;
+        gp(reinterpret_cast<Sample*>(src.vout))[slot] = (Sample)u;
+      }
+      __syncthreads();
+      continue;
+    }
     for (int p = tid; p < src.n; p += blockDim.x) {
       const int64_t cell = base + gp(src.cells)[p];
       if (src.kind == FDTD_SRC_POINT)
@@ -739,6 +774,30 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
         a.F.put(cell, (T)src.amplitude * w);
     }
     __syncthreads();
+  }
+  if (a.is_H) {
+    // detectors.py:388-460, X = 0, Y = 1; numpy wraps index -1 to the last
+    for (int t = 0; t < a.ncur; ++t) {
+      const CurDev& cd = a.curs[t];
+      const gptr<Sample> buf = gp(reinterpret_cast<Sample*>(cd.buf)) + (a.counter - cd.slot0) * cd.n;
+      const T dx = (T)cd.spacing;
+      for (int p = tid; p < cd.n; p += blockDim.x) {
+        const int px = gp(cd.pts)[p], py = gp(cd.pts)[cd.n + p], pz = gp(cd.pts)[2 * cd.n + p];
+        auto H = [&](int c, int x, int y, int z) {
+          x = x < 0 ? x + a.d.nx : x;
+          y = y < 0 ? y + a.d.ny : y;
+          z = z < 0 ? z + a.d.nz : z;
+          return a.F.get(c * a.d.cs + ((int64_t)x * a.d.ny + y) * a.d.nz + z);
+        };
+        T v1 = (H(0, px, py - 1, pz) - H(0, px, py, pz)) * dx;
+        T v2 = (H(1, px, py, pz) - H(1, px - 1, py, pz)) * dx;
+        const T c1 = v1 + v2;
+        v1 = (H(0, px, py - 1, pz - 1) - H(0, px, py, pz - 1)) * dx;
+        v2 = v2 + (H(1, px, py, pz - 1) - H(1, px - 1, py, pz - 1)) * dx;
+        const T c2 = v1 + v2;
+        buf[p] = (Sample)((c1 + c2) / T(2));
+      }
+    }
   }
   if (a.rec) {
     const int64_t nr = box_cells(a.rb);

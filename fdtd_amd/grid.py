@@ -354,6 +354,9 @@ class Grid:
         grown = False
         for det in self.detectors:
             grown |= det._reserve(max(e_steps, h_steps))
+        for src in self.sources:
+            if hasattr(src, "_reserve"):  # sources that record (soft sources)
+                grown |= src._reserve(e_steps)
         if grown:
             plan = self._ensure_plan()
         self._ensure_waveforms(self.time_steps_passed + nsteps)
@@ -394,9 +397,7 @@ class Grid:
         for m in chunks:
             plan = self._prepare(m, m, m)
             plan.run(self.time_steps_passed, m, self._stream())
-            for det in self.detectors:
-                det._count_E += m
-                det._count_H += m
+            self._count_halves(m, m)
             self.time_steps_passed += m
             if bar is not None:
                 bar.update(m)
@@ -414,15 +415,22 @@ class Grid:
         sequence (grid.py:275-299)."""
         plan = self._prepare(1, 1, 0)
         plan.half_step_E(self.time_steps_passed, self._stream())
-        for det in self.detectors:
-            det._count_E += 1
+        self._count_halves(1, 0)
 
     def update_H(self):
         """Everything the reference's update_H does (grid.py:301-325)."""
         plan = self._prepare(1, 0, 1)
         plan.half_step_H(self.time_steps_passed, self._stream())
+        self._count_halves(0, 1)
+
+    def _count_halves(self, e, h):
+        """Samples the device recorded during e E and h H half steps."""
         for det in self.detectors:
-            det._count_H += 1
+            det._count_E += e
+            det._count_H += h
+        for src in self.sources:
+            if hasattr(src, "_count"):
+                src._count += e
 
     def reset(self):
         """zero E, H and the time index; PML state and detector traces are kept

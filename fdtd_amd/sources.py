@@ -262,7 +262,97 @@ class PlaneSource(_SourceBase):
 
 
 class SoftArbitraryPointSource:
-    """Not part of the HIP step path (it needs the CurrentDetector feedback loop)."""
+    """A resistive point source driven by a waveform array (sources.py:504-637).
 
-    def __init__(self, *args, **kwargs):
-        raise NotImplementedError("SoftArbitraryPointSource is not implemented by the HIP backend")
+    In every E half step, on the device: v = waveform_array[q] (0 past its
+    end), I = the current its CurrentDetector measured after the last H half
+    step (0 at q == 0), u = v + impedance * I when impedance > 0, else v;
+    E[x, y, z, 2] += u / grid_spacing. ``input_voltage`` / ``source_voltage``
+    are the reference's per-step lists of v and u. Unlike the reference,
+    naming the source works (its detector is named ``<name>_I``)."""
+
+    def __init__(self, waveform_array, name: str = None, impedance: float = 0.0):
+        self.grid = None
+        self.name = name
+        self.current_detector = None
+        if torch.is_tensor(waveform_array):
+            waveform_array = waveform_array.detach().cpu().numpy()
+        self.waveform_array = np.asarray(waveform_array)
+        self.impedance = impedance
+        self._vin = None
+        self._vout = None
+        self._count = 0
+
+    def _register_grid(self, grid, x: Number, y: Number, z: Number):
+        from .detectors import CurrentDetector
+
+        self.grid = grid
+        self.grid.sources.append(self)
+        if self.name is not None:
+            if not hasattr(grid, self.name):
+                setattr(grid, self.name, self)
+            else:
+                raise ValueError(f"The grid already has an attribute with name {self.name}")
+        try:
+            (x,), (y,), (z,) = x, y, z
+        except (TypeError, ValueError):
+            raise ValueError("a point source should be placed on a single grid cell.")
+        self.x, self.y, self.z = grid._handle_tuple((x, y, z))
+        self._cell = grid._cell(self.x, self.y, self.z)
+        self.current_detector = CurrentDetector(name=None if self.name is None else self.name + "_I")
+        grid[x, y, z] = self.current_detector
+        self._reserve(0)
+
+    def _waveform(self, q):
+        return float(self.waveform_array[q]) if q < self.waveform_array.shape[0] else 0.0
+
+    def _reserve(self, extra):
+        need = self._count + int(extra)
+        cap = 0 if self._vin is None else self._vin.shape[0]
+        if self._vin is not None and need <= cap:
+            return False
+        g = self.grid
+        new_cap = max(need, 2 * cap, 64)
+        vin = torch.zeros(new_cap, dtype=g.dtype, device=g._E.device)
+        vout = torch.zeros(new_cap, dtype=g.dtype, device=g._E.device)
+        if self._vin is not None:
+            vin[: self._count] = self._vin[: self._count]
+            vout[: self._count] = self._vout[: self._count]
+        self._vin, self._vout = vin, vout
+        return True
+
+    def _plan_key(self):
+        return (type(self).__name__, self.impedance, id(self.waveform_array), id(self._vin))
+
+    def _add_to_plan(self, plan):
+        curs = [d for d in self.grid.detectors if type(d).__name__ == "CurrentDetector"]
+        det = next(i for i, d in enumerate(curs) if d is self.current_detector)
+        plan.add_soft_source(self._cell, float(self.impedance), self.grid.grid_spacing, det, self._vin, self._vout,
+                             self._count)
+
+    def trace(self, field="input_voltage"):
+        """All recorded voltages as one device tensor (steps, 1, 1, 1)."""
+        buf = {"input_voltage": self._vin, "source_voltage": self._vout}[field]
+        return buf[: self._count].reshape(self._count, 1, 1, 1)
+
+    @property
+    def input_voltage(self) -> List:
+        t = self.trace("input_voltage")
+        return [t[i] for i in range(t.shape[0])]
+
+    @property
+    def source_voltage(self) -> List:
+        t = self.trace("source_voltage")
+        return [t[i] for i in range(t.shape[0])]
+
+    def update_E(self):
+        raise NotImplementedError("source injection runs inside the HIP step")
+
+    def update_H(self):
+        pass
+
+    def __repr__(self):
+        return f"{self.__class__.__name__}()"
+
+    def __str__(self):
+        return "    " + repr(self) + "\n" + f"        @ x={self.x}, y={self.y}, z={self.z}\n"

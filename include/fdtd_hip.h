@@ -38,7 +38,7 @@ typedef struct FdtdPlan FdtdPlan;
 
 enum { FDTD_F64 = 0, FDTD_F32 = 1, FDTD_F32C = 2 };
 enum { FDTD_OBJ_PLAIN = 0, FDTD_OBJ_ABSORBING = 1, FDTD_OBJ_ANISOTROPIC = 2 };
-enum { FDTD_SRC_POINT = 0, FDTD_SRC_LINE = 1, FDTD_SRC_PLANE = 2 };
+enum { FDTD_SRC_POINT = 0, FDTD_SRC_LINE = 1, FDTD_SRC_PLANE = 2, FDTD_SRC_SOFT = 3 };
 
 /* Version string of the library build. */
 const char* fdtd_version(void);
@@ -114,6 +114,30 @@ int fdtd_plan_set_waveforms(FdtdPlan* plan, const double* table, int64_t capacit
 int fdtd_plan_add_detector(FdtdPlan* plan, int n, const int64_t* cells,
                            void* bufE, void* bufH, int64_t capacity,
                            int64_t count_E, int64_t count_H);
+
+/* Current detector (fdtd/detectors.py:284-476): after every H half step, the
+ * loop current around z at each of n points (host arrays px, py, pz; index -1
+ * wraps to the last plane as in numpy), in the reference's operation order
+ * including its accumulation of the second loop's y-term
+ * (detectors.py:447-460), times `spacing`, stored to bufI[slot * n + p]
+ * (plan dtype; float for FDTD_F32C), slot = samples so far (`count` at
+ * registration). Current detectors are numbered in registration order. */
+int fdtd_plan_add_current_detector(FdtdPlan* plan, int n, const int32_t* px, const int32_t* py, const int32_t* pz,
+                                   double spacing, void* bufI, int64_t capacity, int64_t count);
+
+/* Resistive soft source (fdtd/sources.py:504-637) at one cell, a source in
+ * registration order like the others: in the E half step at time index q,
+ * v = waveform table row of this source at q, I = the last sample of current
+ * detector `detector` (0 when q == 0), u = v + impedance * I if impedance > 0
+ * else v; E_z += u / spacing; v and u are stored to vin / vout[slot] (plan
+ * dtype; float for FDTD_F32C), slot = E half steps so far (`count`). */
+int fdtd_plan_add_soft_source(FdtdPlan* plan, int64_t cell, double impedance, double spacing, int detector,
+                              void* vin, void* vout, int64_t capacity, int64_t count);
+
+/* Samples stored so far by current detector `index` / soft source `index`
+ * (numbered among their kind). */
+int fdtd_plan_current_detector_count(const FdtdPlan* plan, int index, int64_t* count);
+int fdtd_plan_soft_source_count(const FdtdPlan* plan, int index, int64_t* count);
 
 /* Upload the plan tables; required before stepping. */
 int fdtd_plan_finalize(FdtdPlan* plan);

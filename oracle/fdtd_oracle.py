@@ -315,6 +315,58 @@ class _Detector:
         self.H.append(self._sample(self.sim.H))
 
 
+class _CurrentDetector:
+    """Loop current around z at each point of an (inclusive) block, sampled
+    after every H half step (fdtd/detectors.py:284-476). The second loop's
+    y-term is accumulated into the first loop's (the `+=` at
+    detectors.py:456), which the restatement keeps."""
+
+    def __init__(self, sim, xs, ys, zs):
+        self.sim, self.x, self.y, self.z = sim, xs, ys, zs
+        self.I = []
+
+    def _current(self, px, py, pz):
+        H, dx = self.sim.H, self.sim.grid_spacing
+        v1 = (H[px, py - 1, pz, 0] - H[px, py, pz, 0]) * dx
+        v2 = (H[px, py, pz, 1] - H[px - 1, py, pz, 1]) * dx
+        c1 = v1 + v2
+        v1 = (H[px, py - 1, pz - 1, 0] - H[px, py, pz - 1, 0]) * dx
+        v2 += (H[px, py, pz - 1, 1] - H[px - 1, py, pz - 1, 1]) * dx
+        c2 = v1 + v2
+        return (c1 + c2) / 2.0
+
+    def detect_E(self):
+        pass
+
+    def detect_H(self):
+        self.I.append([[[self._current(r, c, p) for p in self.z] for c in self.y] for r in self.x])
+
+
+class _SoftSource:
+    """Resistive point source driven by a waveform array, reading the current
+    its own current detector measured after the previous H half step
+    (fdtd/sources.py:504-637): E_z += (v(q) + R I) / dx."""
+
+    def __init__(self, sim, x, y, z, waveform, impedance):
+        self.sim, self.x, self.y, self.z = sim, x, y, z
+        self.waveform, self.impedance = waveform, impedance
+        self.input_voltage, self.source_voltage = [], []
+        self.current_detector = None
+
+    def update_E(self):
+        s = self.sim
+        q = s.time_steps_passed
+        v_in = self.waveform[q] if q < self.waveform.shape[0] else 0.0
+        current = self.current_detector.I[-1][0][0][0] if q > 0 else 0.0
+        v_out = v_in + self.impedance * current if self.impedance > 0 else v_in
+        s.E[self.x, self.y, self.z, 2] += v_out / s.grid_spacing
+        self.input_voltage.append([[[v_in]]])
+        self.source_voltage.append([[[v_out]]])
+
+    def update_H(self):
+        pass
+
+
 # ---------------------------------------------------------------- the grid
 class Sim:
     """Grid state plus the leap-frog loop (fdtd/grid.py:90-331)."""
@@ -387,7 +439,39 @@ class Sim:
             return self._add_source(kind, x, y, z, **p)
         if kind in ("LineDetector", "BlockDetector"):
             return self._add_detector(kind, x, y, z)
+        if kind == "CurrentDetector":
+            return self._add_current_detector(x, y, z)
+        if kind == "SoftArbitraryPointSource":
+            try:
+                (x,), (y,), (z,) = x, y, z
+            except (TypeError, ValueError):
+                raise ValueError("a point source should be placed on a single grid cell.")
+            src = _SoftSource(self, self.cells_of(x), self.cells_of(y), self.cells_of(z),
+                              np.asarray(p["waveform_array"]), p.get("impedance", 0.0))
+            self.sources.append(src)
+            src.current_detector = self._add_current_detector([src.x], [src.y], [src.z])
+            return src
         raise ValueError(kind)
+
+    def _add_current_detector(self, x, y, z):
+        """Point lists stay as given; otherwise inclusive ranges
+        (detectors.py:330-372)."""
+        if isinstance(x, list) and isinstance(y, list) and isinstance(z, list):
+            if not (len(x) == len(y) == len(z)):
+                raise IndexError("sources require grid to be indexed with slices or equal length list-indices")
+            xs, ys, zs = x, y, z
+        else:
+            r = []
+            for sl, n in ((x, self.Nx), (y, self.Ny), (z, self.Nz)):
+                if isinstance(sl, list):
+                    sl = slice(sl[0], sl[-1])
+                a = sl.start if sl.start is not None else 0
+                b = sl.stop if sl.stop is not None else n
+                r.append(list(range(a, b + 1)))
+            xs, ys, zs = r
+        d = _CurrentDetector(self, xs, ys, zs)
+        self.detectors.append(d)
+        return d
 
     def _claim(self, *faces):
         for f in faces:

@@ -5,7 +5,7 @@ scenarios.py.
 Run once, in the build container only (the reference tree is not available
 anywhere else):
 
-    python tests/golden/make_golden.py /root/reference
+    python tests/golden/make_golden.py /root/reference [scenario ...]
 
 Outputs (data only: inputs are regenerated from scenarios.py and checked
 against the stored checksum):
@@ -96,10 +96,12 @@ def record(fdtd, sc):
             out[p + "z"] = _slice_pair(c.z)
             out[p + "pol"] = np.array([c._Epol, c._Hpol])
             out[p + "period"] = np.int64(c.period)
-        elif cls in ("LineDetector", "BlockDetector"):
+        elif cls in ("LineDetector", "BlockDetector", "CurrentDetector"):
             out[p + "x"] = np.array(c.x)
             out[p + "y"] = np.array(c.y)
             out[p + "z"] = np.array(c.z)
+        elif cls == "SoftArbitraryPointSource":
+            out[p + "xyz"] = np.array([c.x, c.y, c.z])
 
     scenarios.run_program(grid, sc)
 
@@ -127,6 +129,12 @@ def record(fdtd, sc):
         elif cls == "BlockDetector":
             out[p + "E"] = np.array(c.E, dtype=np.float64)
             out[p + "H"] = np.array(c.H, dtype=np.float64)
+        elif cls == "CurrentDetector":
+            out[p + "I"] = np.array(c.I, dtype=np.float64)
+        elif cls == "SoftArbitraryPointSource":
+            out[p + "input_voltage"] = np.array(c.input_voltage, dtype=np.float64)
+            out[p + "source_voltage"] = np.array(c.source_voltage, dtype=np.float64)
+            out[p + "I"] = np.array(c.current_detector.I, dtype=np.float64)
     return out
 
 
@@ -153,13 +161,16 @@ def curl_golden(ref_root):
     }
 
 
-def main(ref_root):
+def main(ref_root, only=None):
     sys.path.insert(0, ref_root)
     import fdtd  # the reference package
 
     fdtd.set_backend("numpy")
-    np.savez_compressed(os.path.join(HERE, "curl_golden.npz"), **curl_golden(ref_root))
+    if not only:
+        np.savez_compressed(os.path.join(HERE, "curl_golden.npz"), **curl_golden(ref_root))
     for sc in scenarios.all_scenarios():
+        if only and sc["name"] not in only:
+            continue
         out = record(fdtd, sc)
         fn = os.path.join(HERE, f"{sc['name']}.npz")
         np.savez_compressed(fn, **out)
@@ -167,4 +178,5 @@ def main(ref_root):
 
 
 if __name__ == "__main__":
-    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference")
+    # make_golden.py [reference root] [scenario names...]: all scenarios by default
+    main(sys.argv[1] if len(sys.argv) > 1 else "/root/reference", sys.argv[2:])

@@ -222,8 +222,49 @@ def scenario_1d():
     )
 
 
+def scenario_soft_source():
+    """A probe-fed patch in miniature (after the reference's antenna test,
+    tests/test_antenna_impedance.py:18-95): copper planes as absorbing
+    objects, a dielectric substrate, a resistive soft source (Gaussian pulse
+    waveform) that reads its own loop current every step, and a stand-alone
+    current detector over a block. The impedance is 0.2 in the grid's
+    normalised units: the source feeds R*I/dx back into E every step, and
+    the loop gain ~2.3 R must stay below 1 for the run to stay bounded."""
+    s = slice
+    steps = 60
+    dx = 0.4e-3
+    t = np.arange(steps) * (0.99 * 3 ** -0.5 * dx / 299792458.0)
+    wave = _gaussian(t, 12 * t[1], 20 * t[1])
+    copper = dict(permittivity=1.0, conductivity=1e8)
+    return dict(
+        name="soft_source",
+        shape=(20, 16, 14),
+        grid_spacing=dx,
+        courant_number=None,
+        permittivity=1.0,
+        permeability=1.0,
+        components=[
+            ("PML", dict(), (s(0, 3), s(None), s(None)), {}),
+            ("PML", dict(), (s(-3, None), s(None), s(None)), {}),
+            ("PML", dict(), (s(None), s(0, 3), s(None)), {}),
+            ("PML", dict(), (s(None), s(-3, None), s(None)), {}),
+            ("PML", dict(), (s(None), s(None), s(0, 3)), {}),
+            ("PML", dict(), (s(None), s(None), s(-3, None)), {}),
+            ("AbsorbingObject", copper, (s(3, -3), s(3, -3), s(3, 4)), {}),
+            ("AbsorbingObject", copper, (s(3, -3), s(3, -3), s(7, 8)), {}),
+            ("SoftArbitraryPointSource", dict(waveform_array=wave, impedance=0.2), (9, 8, 4), {}),
+            ("AbsorbingObject", copper, (s(9, 10), s(8, 9), s(5, 8)), {}),
+            ("Object", dict(permittivity=2.42), (s(3, 9), s(3, -3), s(4, 7)), {}),
+            ("Object", dict(permittivity=2.42), (s(10, -3), s(3, -3), s(4, 7)), {}),
+            ("CurrentDetector", dict(name="loop"), (s(5, 7), s(5, 6), s(5, 5)), {}),
+        ],
+        program=[("step", steps)],
+    )
+
+
 def all_scenarios():
-    out = [scenario_c1(), scenario_g2(), scenario_manual(), scenario_float_keys(), scenario_1d()]
+    out = [scenario_c1(), scenario_g2(), scenario_manual(), scenario_float_keys(), scenario_1d(),
+           scenario_soft_source()]
     for axis in range(3):
         for first in (True, False):
             out.append(_periodic_scenario(f"periodic_{'xyz'[axis]}_{'first' if first else 'last'}", axis, first))
@@ -273,7 +314,8 @@ def build_fdtd(mod, sc, to_array=None):
     comps = []
     for kind, params, key, overrides in sc["components"]:
         p = {k: (conv(v) if isinstance(v, np.ndarray) else v) for k, v in params.items()}
-        comp = getattr(mod, kind)(**p)
+        cls = getattr(mod, kind, None) or getattr(mod.sources, kind)  # some live in submodules only
+        comp = cls(**p)
         grid[key] = comp
         for attr, val in overrides.items():
             setattr(comp, attr, conv(val) if isinstance(val, np.ndarray) else val)

@@ -82,6 +82,12 @@ def test_float64_matches_reference_fixture(golden, name):
         elif cls in ("LineDetector", "BlockDetector"):
             _assert_same(_np(c.trace("E")), g[p + "E"], f"detector {i} E")
             _assert_same(_np(c.trace("H")), g[p + "H"], f"detector {i} H")
+        elif cls == "CurrentDetector":
+            _assert_same(_np(c.trace("I")), g[p + "I"], f"current detector {i}")
+        elif cls == "SoftArbitraryPointSource":
+            _assert_same(_np(c.trace("input_voltage")), g[p + "input_voltage"], f"source {i} input voltage")
+            _assert_same(_np(c.trace("source_voltage")), g[p + "source_voltage"], f"source {i} source voltage")
+            _assert_same(_np(c.current_detector.trace("I")), g[p + "I"], f"source {i} current")
 
 
 @pytest.mark.parametrize("seed", range(16))

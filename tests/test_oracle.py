@@ -68,8 +68,10 @@ def test_oracle_matches_reference_fixture(golden, name):
             assert [c.x, c.y, c.z] == list(g[p + "xyz"]) and c.period == int(g[p + "period"])
         elif cls == "PlaneSource":
             assert [c.Epol, c.Hpol] == list(g[p + "pol"])
-        elif cls in ("LineDetector", "BlockDetector"):
+        elif cls in ("LineDetector", "BlockDetector", "CurrentDetector"):
             assert list(c.x) == list(g[p + "x"]) and list(c.y) == list(g[p + "y"]) and list(c.z) == list(g[p + "z"])
+        elif cls == "SoftArbitraryPointSource":
+            assert [c.x, c.y, c.z] == list(g[p + "xyz"])
 
     scenarios.run_program(sim, sc)
     assert sim.time_steps_passed == int(g["time_steps_passed"])
@@ -100,6 +102,12 @@ def test_oracle_matches_reference_fixture(golden, name):
             tol = 1e-13 * max(np.abs(g[p + "E"]).max(), 1e-30) if has_aniso else 0
             np.testing.assert_allclose(e, g[p + "E"], rtol=0, atol=tol)
             np.testing.assert_allclose(h, g[p + "H"], rtol=0, atol=tol)
+        elif cls == "CurrentDetector":
+            assert np.array_equal(np.array(c.I), g[p + "I"])
+        elif cls == "SoftArbitraryPointSource":
+            assert np.array_equal(np.array(c.input_voltage), g[p + "input_voltage"])
+            assert np.array_equal(np.array(c.source_voltage), g[p + "source_voltage"])
+            assert np.array_equal(np.array(c.current_detector.I), g[p + "I"])
 
 
 def test_courant_and_time_step_closed_forms():
