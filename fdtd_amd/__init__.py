@@ -21,3 +21,4 @@ from .boundaries import PeriodicBoundary, PML, DomainBorderPML
 from .backend import backend
 from .backend import set_backend
 from .backend import backend_names_available
+from .fourier import FrequencyRoutines

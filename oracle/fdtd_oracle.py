@@ -677,3 +677,28 @@ class Sim:
         self.H *= 0.0
         self.E *= 0.0
         self.time_steps_passed *= 0
+
+
+# ------------------------------------------------------ frequency routines
+# fdtd/fourier.py:52-266 with the numpy backend and default arguments: the
+# history padded with its last value ('edge') to the bin count the default
+# window asks for, numpy's FFT, bins [0, -1).
+def fr_padding(n, dt):
+    end_freq = (n / 2.0) / (dt * n)
+    return math.ceil(n / ((end_freq - 0) * dt)) - n
+
+
+def fr_fft(history, dt):
+    x = np.asarray(history, dtype=np.float64).reshape(-1)
+    x = np.pad(x, (0, fr_padding(x.shape[0], dt)), "edge")
+    return np.fft.fftfreq(x.shape[0], d=dt)[0:-1], np.fft.fft(x)[0:-1]
+
+
+def fr_impedance(voltage, current, dt, steps):
+    v = np.asarray(voltage, dtype=np.float64)
+    pad = fr_padding(v.shape[0], dt)
+    v = np.pad(np.reshape(v, steps), (0, pad), "edge")
+    i = np.pad(np.reshape(np.asarray(current, dtype=np.float64), steps), (0, pad), "edge")
+    vs, cs = np.fft.fft(v), np.fft.fft(i)
+    z = np.divide(vs, cs, out=np.zeros_like(vs), where=cs != 0)
+    return np.fft.fftfreq(v.shape[0], d=dt)[0:-1], z[0:-1]

@@ -135,6 +135,13 @@ def record(fdtd, sc):
             out[p + "input_voltage"] = np.array(c.input_voltage, dtype=np.float64)
             out[p + "source_voltage"] = np.array(c.source_voltage, dtype=np.float64)
             out[p + "I"] = np.array(c.current_detector.I, dtype=np.float64)
+            # the reference's frequency routines on this run (fourier.py:172-266)
+            fr = fdtd.FrequencyRoutines(grid, c)
+            f, z = fr.impedance()
+            out[p + "impedance_freqs"], out[p + "impedance"] = np.asarray(f), np.asarray(z)
+            hist = np.array(c.source_voltage, dtype=np.float64).reshape(-1)
+            f, sp = fdtd.FrequencyRoutines(grid, hist).FFT()
+            out[p + "fft_freqs"], out[p + "fft"] = np.asarray(f), np.asarray(sp)
     return out
 
 

@@ -284,3 +284,28 @@ def test_compensated_float32_error_against_float64_reference(golden, name):
         cls = str(g[f"reg/{i}/class"])
         if cls in ("LineDetector", "BlockDetector"):
             assert rel_l2(_np(c.trace("E")), g[f"out/{i}/E"]) < 1e-6
+
+
+def rel_c(a, b):
+    """Relative L2 distance of two complex arrays."""
+    a, b = np.asarray(a, np.complex128), np.asarray(b, np.complex128)
+    return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
+
+
+def test_frequency_routines_match_reference(golden):
+    """FrequencyRoutines on the device (torch.fft, complex128) after the
+    soft-source run, against what the reference's routines computed on its
+    own run (numpy FFT): equal to 1e-12 relative."""
+    g = golden("soft_source")
+    sc = scenarios.get_scenario("soft_source")
+    grid, comps = _build(sc)
+    scenarios.run_program(grid, sc)
+    src = comps[8]
+    f, z = fdtd.FrequencyRoutines(grid, src).impedance()
+    assert rel_l2(_np(f), g["out/8/impedance_freqs"]) < 1e-15
+    assert rel_c(_np(z), g["out/8/impedance"]) < 1e-12
+    hist = src.trace("source_voltage").reshape(-1)
+    f, sp = fdtd.FrequencyRoutines(grid, hist).FFT()
+    assert rel_l2(_np(f), g["out/8/fft_freqs"]) < 1e-15
+    assert rel_c(_np(sp), g["out/8/fft"]) < 1e-12
+    assert fdtd.FrequencyRoutines(fdtd.Grid((8, 8, 8), grid_spacing=1e-3), hist).FFT() == ([], [])

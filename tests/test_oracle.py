@@ -161,3 +161,15 @@ def test_pml_sigma_profile_closed_form():
     assert np.allclose(f.bE[:, 0, 0, 0], b, rtol=1e-15)
     # the H-side profile ends with a zero-conductivity cell
     assert f.sigmaH[-1, 0, 0, 0] == 0.0 and f.cH[-1, 0, 0, 0] == 0.0
+
+
+def test_frequency_routines_match_reference_fixture(golden):
+    """The impedance and FFT the reference's FrequencyRoutines computed on
+    the soft-source run, restated with numpy (fourier.py:52-266)."""
+    g = golden("soft_source")
+    dt = float(g["reg/time_step"])
+    steps = int(g["time_steps_passed"])
+    f, z = orc.fr_impedance(g["out/8/source_voltage"], g["out/8/I"], dt, steps)
+    assert np.array_equal(f, g["out/8/impedance_freqs"]) and np.array_equal(z, g["out/8/impedance"])
+    f, sp = orc.fr_fft(g["out/8/source_voltage"], dt)
+    assert np.array_equal(f, g["out/8/fft_freqs"]) and np.array_equal(sp, g["out/8/fft"])
