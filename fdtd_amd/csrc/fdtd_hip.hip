@@ -205,7 +205,7 @@ PlainField<float, float> make_field(const FdtdPlan* p, bool H) {
 }
 template <>
 CompField make_field(const FdtdPlan* p, bool H) {
-  return {reinterpret_cast<float*>(H ? p->H : p->E), reinterpret_cast<uint32_t*>(H ? p->H_lo : p->E_lo), p->d.cs};
+  return {reinterpret_cast<float*>(H ? p->H : p->E), reinterpret_cast<float*>(H ? p->H_lo : p->E_lo), p->d.cs};
 }
 template <class MS>
 MS make_mat(const FdtdPlan* p, bool mu);
@@ -250,7 +250,9 @@ template <class FS, class MS, typename O>
 void launch_main(const FdtdPlan* p, const MainArgs<FS, MS>& a, bool is_H, hipStream_t st) {
   const int bz = 1 << a.bz_shift, by = a.by;
   const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
-  const dim3 grid(tiles * (unsigned)((p->d.nx + a.cx - 1) / a.cx)), block(bz * by);
+  const unsigned nchunks = (unsigned)((p->d.nx + a.cx - 1) / a.cx);
+  // x-chunks dealt cyclically to the 8 XCDs (xcd_chunk_tile): whole rounds of 8
+  const dim3 grid(tiles * ((nchunks + 7) / 8 * 8)), block(bz * by);
   if (a.rec) {  // recording forward pass of a gradient (plain storage)
     if (a.nobjs)
       hipLaunchKernelGGL((k_main_E<FS, MS, O, true, true, true>), grid, block, 0, st, a);

@@ -176,41 +176,47 @@ __device__ __forceinline__ float comp_hi(double v, uint32_t& q) {
 
 // The floats are (3, N) like a plain field; the three compensation bytes of
 // a cell share one 32-bit word (byte c for component c), so a cell's
-// components cost one extra 4-byte access, not three byte accesses.
+// components cost one extra 4-byte access, not three byte accesses. The words
+// are accessed as float bit patterns (moved, never computed with): stores of
+// an integer type could alias the integer fields of the face and object
+// tables that the same loops read, and type-based alias analysis would then
+// turn those uniform table reads into vector loads that wait for every
+// outstanding store, once per face and cell.
 struct CompField {
   using R = double;
   using Sample = float;
-  float* hi;     // (3, N)
-  uint32_t* lo;  // (N)
+  float* hi;  // (3, N)
+  float* lo;  // (N) words
   int64_t cs;
   __device__ __forceinline__ int comp_of(int64_t i) const { return i >= 2 * cs ? 2 : (i >= cs ? 1 : 0); }
+  __device__ __forceinline__ uint32_t word(int64_t cell) const { return __float_as_uint(gp(lo)[cell]); }
+  __device__ __forceinline__ void set_byte(int64_t cell, int c, uint32_t q) const {
+    gp(lo)[cell] = __uint_as_float((word(cell) & ~(0xffu << (8 * c))) | (q << (8 * c)));
+  }
   __device__ __forceinline__ double get(int64_t i) const {
     const int c = comp_of(i);
-    return comp_decode(gp(hi)[i], (gp(lo)[i - c * cs] >> (8 * c)) & 0xffu);
+    return comp_decode(gp(hi)[i], (word(i - c * cs) >> (8 * c)) & 0xffu);
   }
   __device__ __forceinline__ void put(int64_t i, double v) const {
     const int c = comp_of(i);
     uint32_t q;
     gp(hi)[i] = comp_hi(v, q);
-    const gptr<uint32_t> w = gp(lo) + (i - c * cs);
-    *w = (*w & ~(0xffu << (8 * c))) | (q << (8 * c));
+    set_byte(i - c * cs, c, q);
   }
   __device__ __forceinline__ void copy(int64_t dst, int64_t src) const {
-    const int c = comp_of(dst);
+    const int cd = comp_of(dst), cs_ = comp_of(src);
     gp(hi)[dst] = gp(hi)[src];
-    const uint32_t q = (gp(lo)[src - comp_of(src) * cs] >> (8 * comp_of(src))) & 0xffu;
-    const gptr<uint32_t> w = gp(lo) + (dst - c * cs);
-    *w = (*w & ~(0xffu << (8 * c))) | (q << (8 * c));
+    set_byte(dst - cd * cs, cd, (word(src - cs_ * cs) >> (8 * cs_)) & 0xffu);
   }
   // detectors sample the value rounded to float
   __device__ __forceinline__ float sample(int64_t i) const { return (float)get(i); }
   template <int C>
   __device__ __forceinline__ double get_c(int64_t cell) const {
-    return comp_decode(gp(hi)[C * cs + cell], (gp(lo)[cell] >> (8 * C)) & 0xffu);
+    return comp_decode(gp(hi)[C * cs + cell], (word(cell) >> (8 * C)) & 0xffu);
   }
   template <bool NT>
   __device__ __forceinline__ void get3_s(int64_t cell, double v[3]) const {
-    const uint32_t w = ld_stream<NT>(lo + cell);
+    const uint32_t w = __float_as_uint(ld_stream<NT>(lo + cell));
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = comp_decode(ld_stream<NT>(hi + c * cs + cell), (w >> (8 * c)) & 0xffu);
   }
@@ -223,7 +229,7 @@ struct CompField {
       st_stream<NT>(hi + c * cs + cell, comp_hi(v[c], q));
       w |= q << (8 * c);
     }
-    st_stream<NT>(lo + cell, w);
+    st_stream<NT>(lo + cell, __uint_as_float(w));
   }
 };
 
@@ -315,11 +321,33 @@ __device__ __forceinline__ void pml_face(const FaceDev& fd, const Dims& d, int i
   v2 = n2;
 }
 
-// All faces in `faces` that contain cell (i,j,k), in registration order.
+// The faces among the first 64 whose slab meets the block [x0,x1) x [y0,y1)
+// x [z0,z1) of cells, as a bit mask (workgroup-uniform). A tile in the
+// interior meets none and skips the face loop altogether.
+__device__ __forceinline__ uint64_t faces_meeting(const FaceDev* __restrict__ faces, int nfaces, const Dims& d, int x0,
+                                                  int x1, int y0, int y1, int z0, int z1) {
+  uint64_t m = 0;
+  for (int f = 0; f < nfaces && f < 64; ++f) {
+    const FaceDev& fd = faces[f];
+    const int ax = fd.axis;
+    const int n = ax == 0 ? d.nx : (ax == 1 ? d.ny : d.nz);
+    const int lo = ax == 0 ? x0 : (ax == 1 ? y0 : z0), hi = ax == 0 ? x1 : (ax == 1 ? y1 : z1);
+    const int s0 = fd.high ? n - fd.t : 0, s1 = fd.high ? n : fd.t;
+    if (lo < s1 && s0 < hi) m |= 1ull << f;
+  }
+  return m;
+}
+
+// The faces of `mask` (then every face past the 64th) that contain cell
+// (i,j,k), applied in registration order.
 template <bool IS_E, typename T>
-__device__ __forceinline__ void pml_cell(const FaceDev* __restrict__ faces, int nfaces, const Dims& d, int i, int j,
-                                         int k, const Diffs<T>& df, const T kc[3], T v[3]) {
-  for (int fi = 0; fi < nfaces; ++fi)
+__device__ __forceinline__ void pml_cell_masked(const FaceDev* __restrict__ faces, uint64_t mask, int nfaces,
+                                                const Dims& d, int i, int j, int k, const Diffs<T>& df, const T kc[3],
+                                                T v[3]) {
+  for (uint64_t m = mask; m; m &= m - 1)
+    pml_face<IS_E>(faces[__builtin_ctzll(m)], d, i, j, k, df.zx, df.yx, df.zy, df.xy, df.yz, df.xz, kc[0], kc[1],
+                   kc[2], v[0], v[1], v[2]);
+  for (int fi = 64; fi < nfaces; ++fi)
     pml_face<IS_E>(faces[fi], d, i, j, k, df.zx, df.yx, df.zy, df.xy, df.yz, df.xz, kc[0], kc[1], kc[2], v[0], v[1],
                    v[2]);
 }
@@ -364,6 +392,20 @@ __device__ __forceinline__ void objects_update_E(const ObjDev* __restrict__ objs
 __device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned nb) {
   const unsigned per = nb / 8, rem = nb % 8, x = w % 8, idx = w / 8;
   return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
+}
+
+// Tile of workgroup w in the single-grid kernels, or -1 for an idle
+// workgroup. The x-chunks (`per_chunk` tiles each) are dealt to the XCDs
+// cyclically -- chunk c to XCD c mod 8 -- and each XCD walks the tiles of its
+// chunks in order, so at any moment the 8 XCDs work on 8 neighbouring chunks:
+// the CPML slabs at both x ends, which cost more per cell, are shared by all
+// XCDs instead of falling to the first and last (workgroups are dispatched in
+// order, so one slow XCD holds back the others). The grid has
+// 8 * ceil(nchunks / 8) * per_chunk workgroups.
+__device__ __forceinline__ int64_t xcd_chunk_tile(unsigned w, unsigned per_chunk, unsigned nchunks) {
+  const unsigned x = w % 8, k = w / 8;
+  const unsigned tx = (k / per_chunk) * 8 + x;
+  return tx < nchunks ? (int64_t)tx * per_chunk + k % per_chunk : -1;
 }
 
 // Half-open box of cells {x0, x1, y0, y1, z0, z1}; its cells are numbered
@@ -423,60 +465,72 @@ struct MainArgs {
 // Main E kernel. A workgroup owns a (y, z) tile of bz*by columns (z fastest,
 // one row of bz cells per wavefront when bz = 64) and marches along x over a
 // chunk of `cx` planes: the x-neighbours come from registers of the previous
-// plane, the z-neighbours from the neighbouring lane, the y-neighbours from
-// cache.
-template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC = false>
-__device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile, int64_t q) {
+// plane, the y- and z-neighbours from cache (the rows were just streamed in
+// by this or the neighbouring wavefront). Every load of a plane is issued
+// unconditionally and at once, so a plane costs one memory round trip:
+// threads past the grid's edge read the nearest cell and store nothing, and
+// at the low edges the neighbour offset is 0 (the difference is dropped).
+// FULL: every thread of the tile is inside the grid. The body is then free of
+// lane-dependent branches around the stores, which would otherwise let the
+// compiler sink the plane's loads behind them (two round trips per plane).
+template <class FS, class MS>
+__device__ __forceinline__ bool tile_full(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
+  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
+  const int ntz = (a.d.nz + bz - 1) >> bzs, nty = (a.d.ny + by - 1) / by;
+  const int tz = tile % ntz, ty = (tile / ntz) % nty;
+  return (tz + 1) * bz <= a.d.nz && (ty + 1) * by <= a.d.ny;
+}
+
+template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC, bool FULL>
+__device__ __forceinline__ void main_E_body(const MainArgs<FS, MS>& __restrict__ a, unsigned tile, int64_t q) {
   using T = typename FS::R;
   const Dims d = a.d;
   const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
   const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
   const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
-  const int lz = threadIdx.x & (bz - 1);
-  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int k0 = tz * bz + (threadIdx.x & (bz - 1)), j0 = ty * by + (threadIdx.x >> bzs);
+  const bool act = k0 < d.nz && j0 < d.ny;
+  const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
   const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
-  const bool act = k < d.nz && j < d.ny;
-  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz;
+  const int64_t zm = k > 0 ? 1 : 0, ym = j > 0 ? sy : 0;
   const T zero = T(0);
+  const uint64_t fmask = a.nfaces ? faces_meeting(a.faces, a.nfaces, d, i0, i1, ty * by, min(ty * by + by, d.ny),
+                                                  tz * bz, min(tz * bz + bz, d.nz))
+                                  : 0;
   int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
   T hz_prev = zero, hy_prev = zero;
-  if (act && i0 > 0) {
+  if (i0 > 0) {
     hz_prev = a.G.template get_c<2>(cell - sx);
     hy_prev = a.G.template get_c<1>(cell - sx);
   }
   for (int i = i0; i < i1; ++i, cell += sx) {
-    T hx = zero, hy = zero, hz = zero;
-    if (act) {
-      T hv[3];
-      a.G.template get3_s<false>(cell, hv);
-      hx = hv[0];
-      hy = hv[1];
-      hz = hv[2];
+    T hv[3], m[3], e[3];
+    a.G.template get3_s<false>(cell, hv);
+    const T hy_km = a.G.template get_c<1>(cell - zm), hx_km = a.G.template get_c<0>(cell - zm);
+    const T hz_jm = a.G.template get_c<2>(cell - ym), hx_jm = a.G.template get_c<0>(cell - ym);
+    a.M.template get3<NT>(cell, m);
+    a.F.template get3_s<NT>(cell, e);
+    const T hx = hv[0], hy = hv[1], hz = hv[2];
+    // backward differences; an absent one contributes nothing (grid.py:67-74)
+    Diffs<T> df;
+    df.zy = j > 0 ? hz - hz_jm : zero;
+    df.yz = k > 0 ? hy - hy_km : zero;
+    df.xz = k > 0 ? hx - hx_km : zero;
+    df.zx = i > 0 ? hz - hz_prev : zero;
+    df.yx = i > 0 ? hy - hy_prev : zero;
+    df.xy = j > 0 ? hx - hx_jm : zero;
+    T curl[3];
+    curl[0] = (zero + df.zy) - df.yz;
+    curl[1] = (zero + df.xz) - df.zx;
+    curl[2] = (zero + df.yx) - df.xy;
+    T kc[3];
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+      kc[c] = a.sc * m[c];
+      e[c] = e[c] + kc[c] * curl[c];
     }
-    T hy_km = __shfl_up(hy, 1);
-    T hx_km = __shfl_up(hx, 1);
-    if (act) {
-      if (lz == 0 && k > 0) {
-        hy_km = a.G.template get_c<1>(cell - 1);
-        hx_km = a.G.template get_c<0>(cell - 1);
-      }
-      T hz_jm = zero, hx_jm = zero;
-      if (j > 0) {
-        hz_jm = a.G.template get_c<2>(cell - sy);
-        hx_jm = a.G.template get_c<0>(cell - sy);
-      }
-      // backward differences; an absent one contributes nothing (grid.py:67-74)
-      Diffs<T> df;
-      df.zy = j > 0 ? hz - hz_jm : zero;
-      df.yz = k > 0 ? hy - hy_km : zero;
-      df.xz = k > 0 ? hx - hx_km : zero;
-      df.zx = i > 0 ? hz - hz_prev : zero;
-      df.yx = i > 0 ? hy - hy_prev : zero;
-      df.xy = j > 0 ? hx - hx_jm : zero;
-      T curl[3];
-      curl[0] = (zero + df.zy) - df.yz;
-      curl[1] = (zero + df.xz) - df.zx;
-      curl[2] = (zero + df.yx) - df.xy;
+    if (FULL || act) {
       if (REC) {
         const int64_t r = box_index(a.rb, i, j, k);
         if (r >= 0) {
@@ -485,16 +539,8 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
           for (int c = 0; c < 3; ++c) a.rec[c * nr + r] = curl[c];
         }
       }
-      T m[3], kc[3], e[3];
-      a.M.template get3<NT>(cell, m);
-      a.F.template get3_s<NT>(cell, e);
-#pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        kc[c] = a.sc * m[c];
-        e[c] = e[c] + kc[c] * curl[c];
-      }
       if (OBJS) objects_update_E<T, O>(a.objs, a.nobjs, a.sc, i, j, k, curl, e);
-      if (a.nfaces) pml_cell<true>(a.faces, a.nfaces, d, i, j, k, df, kc, e);
+      if (fmask || a.nfaces > 64) pml_cell_masked<true>(a.faces, fmask, a.nfaces, d, i, j, k, df, kc, e);
       for (int s = 0; s < a.npts; ++s)  // sources.py:109, registration order
         if (a.pt_cell[s] == cell) e[2] = e[2] + (T)gp(a.wave)[s * a.wave_cap + q];
       a.F.template put3_s<NT>(cell, e);
@@ -504,77 +550,84 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
   }
 }
 
+template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC = false>
+__device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile, int64_t q) {
+  if (tile_full(a, tile)) main_E_body<FS, MS, O, OBJS, NT, REC, true>(a, tile, q);
+  else main_E_body<FS, MS, O, OBJS, NT, REC, false>(a, tile, q);
+}
+
+// tiles per x-chunk and the number of x-chunks of a single-grid launch
+template <class FS, class MS>
+__device__ __forceinline__ void chunk_counts(const MainArgs<FS, MS>& a, unsigned& per_chunk, unsigned& nchunks) {
+  const int bz = 1 << a.bz_shift;
+  per_chunk = (unsigned)(((a.d.nz + bz - 1) / bz) * ((a.d.ny + a.by - 1) / a.by));
+  nchunks = (unsigned)((a.d.nx + a.cx - 1) / a.cx);
+}
+
 template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC>
 __global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
-  main_E_tile<FS, MS, O, OBJS, NT, REC>(a, xcd_tile(blockIdx.x, gridDim.x), a.q);
+  unsigned per, nch;
+  chunk_counts(a, per, nch);
+  const int64_t t = xcd_chunk_tile(blockIdx.x, per, nch);
+  if (t >= 0) main_E_tile<FS, MS, O, OBJS, NT, REC>(a, (unsigned)t, a.q);
 }
 
 // Main H kernel: the mirror image of the E kernel. Its differences are
 // forward ones, so workgroups take the tiles in decreasing order (the +y/+z
 // neighbour tiles were read just before, by the same XCD) and march along -x,
 // keeping E of plane i+1 in registers.
-template <class FS, class MS, bool NT>
-__device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
+template <class FS, class MS, bool NT, bool FULL>
+__device__ __forceinline__ void main_H_body(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
   using T = typename FS::R;
   const Dims d = a.d;
   const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
   const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
   const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
-  const int lz = threadIdx.x & (bz - 1);
-  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int k0 = tz * bz + (threadIdx.x & (bz - 1)), j0 = ty * by + (threadIdx.x >> bzs);
+  const bool act = k0 < d.nz && j0 < d.ny;
+  const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
   const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
-  const bool act = k < d.nz && j < d.ny;
-  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz;
+  const int64_t zp = k < d.nz - 1 ? 1 : 0, yp = j < d.ny - 1 ? sy : 0;
   const T zero = T(0);
+  const uint64_t fmask = a.nfaces ? faces_meeting(a.faces, a.nfaces, d, i0, i1, ty * by, min(ty * by + by, d.ny),
+                                                  tz * bz, min(tz * bz + bz, d.nz))
+                                  : 0;
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   T ez_next = zero, ey_next = zero;  // E at plane i+1
-  if (act && i1 < d.nx) {
+  if (i1 < d.nx) {
     ez_next = a.G.template get_c<2>(cell + sx);
     ey_next = a.G.template get_c<1>(cell + sx);
   }
   for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
-    T ex = zero, ey = zero, ez = zero;
-    if (act) {
-      T ev[3];
-      a.G.template get3_s<false>(cell, ev);
-      ex = ev[0];
-      ey = ev[1];
-      ez = ev[2];
-    }
-    T ey_kp = __shfl_down(ey, 1);
-    T ex_kp = __shfl_down(ex, 1);
-    if (act) {
-      if (lz == bz - 1 && k < d.nz - 1) {
-        ey_kp = a.G.template get_c<1>(cell + 1);
-        ex_kp = a.G.template get_c<0>(cell + 1);
-      }
-      T ez_jp = zero, ex_jp = zero;
-      if (j < d.ny - 1) {
-        ez_jp = a.G.template get_c<2>(cell + sy);
-        ex_jp = a.G.template get_c<0>(cell + sy);
-      }
-      // forward differences; an absent one contributes nothing (grid.py:42-49)
-      Diffs<T> df;
-      df.zy = j < d.ny - 1 ? ez_jp - ez : zero;
-      df.yz = k < d.nz - 1 ? ey_kp - ey : zero;
-      df.xz = k < d.nz - 1 ? ex_kp - ex : zero;
-      df.zx = i < d.nx - 1 ? ez_next - ez : zero;
-      df.yx = i < d.nx - 1 ? ey_next - ey : zero;
-      df.xy = j < d.ny - 1 ? ex_jp - ex : zero;
-      T curl[3];
-      curl[0] = (zero + df.zy) - df.yz;
-      curl[1] = (zero + df.xz) - df.zx;
-      curl[2] = (zero + df.yx) - df.xy;
-      T m[3], kc[3], h[3];
-      a.M.template get3<NT>(cell, m);
-      a.F.template get3_s<NT>(cell, h);
+    T ev[3], m[3], h[3];
+    a.G.template get3_s<false>(cell, ev);
+    const T ey_kp = a.G.template get_c<1>(cell + zp), ex_kp = a.G.template get_c<0>(cell + zp);
+    const T ez_jp = a.G.template get_c<2>(cell + yp), ex_jp = a.G.template get_c<0>(cell + yp);
+    a.M.template get3<NT>(cell, m);
+    a.F.template get3_s<NT>(cell, h);
+    const T ex = ev[0], ey = ev[1], ez = ev[2];
+    // forward differences; an absent one contributes nothing (grid.py:42-49)
+    Diffs<T> df;
+    df.zy = j < d.ny - 1 ? ez_jp - ez : zero;
+    df.yz = k < d.nz - 1 ? ey_kp - ey : zero;
+    df.xz = k < d.nz - 1 ? ex_kp - ex : zero;
+    df.zx = i < d.nx - 1 ? ez_next - ez : zero;
+    df.yx = i < d.nx - 1 ? ey_next - ey : zero;
+    df.xy = j < d.ny - 1 ? ex_jp - ex : zero;
+    T curl[3];
+    curl[0] = (zero + df.zy) - df.yz;
+    curl[1] = (zero + df.xz) - df.zx;
+    curl[2] = (zero + df.yx) - df.xy;
+    T kc[3];
 #pragma unroll
-      for (int c = 0; c < 3; ++c) {
-        kc[c] = a.sc * m[c];
-        h[c] = h[c] - kc[c] * curl[c];
-      }
+    for (int c = 0; c < 3; ++c) {
+      kc[c] = a.sc * m[c];
+      h[c] = h[c] - kc[c] * curl[c];
+    }
+    if (FULL || act) {
       // objects leave H unchanged (objects.py:131-137, 223-229, 271-277)
-      if (a.nfaces) pml_cell<false>(a.faces, a.nfaces, d, i, j, k, df, kc, h);
+      if (fmask || a.nfaces > 64) pml_cell_masked<false>(a.faces, fmask, a.nfaces, d, i, j, k, df, kc, h);
       a.F.template put3_s<NT>(cell, h);
     }
     ez_next = ez;
@@ -583,8 +636,17 @@ __device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__
 }
 
 template <class FS, class MS, bool NT>
+__device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
+  if (tile_full(a, tile)) main_H_body<FS, MS, NT, true>(a, tile);
+  else main_H_body<FS, MS, NT, false>(a, tile);
+}
+
+template <class FS, class MS, bool NT>
 __global__ void __launch_bounds__(1024) k_main_H(MainArgs<FS, MS> a) {
-  main_H_tile<FS, MS, NT>(a, gridDim.x - 1 - xcd_tile(blockIdx.x, gridDim.x));
+  unsigned per, nch;
+  chunk_counts(a, per, nch);
+  const int64_t t = xcd_chunk_tile(blockIdx.x, per, nch);
+  if (t >= 0) main_H_tile<FS, MS, NT>(a, (unsigned)((int64_t)per * nch - 1 - t));
 }
 
 // ------------------------------------------------------------ batched kernels
