@@ -209,6 +209,53 @@ class IntensityGradient:
         gradient has the shape of ``permittivity``, float64."""
         return self._run(permittivity, gradient=True)
 
+    def _reverse_checkpointed(self, adj, segments, checkpoints, curl, plane):
+        """Reverse sweep of the checkpointed mode on two streams: segment s-1
+        is recomputed from its checkpoint (forward kernels, recording) on a
+        second stream while the transposed steps of segment s run on the
+        current one. Both are latency-bound kernels on disjoint data (the
+        recomputation writes the grid state and one record buffer, the
+        transposed steps read the other buffer and write the adjoint state),
+        so they overlap; events order each buffer's reuse."""
+        cur = torch.cuda.current_stream()
+        side = getattr(self, "_side", None)
+        if side is None or side.device != cur.device:
+            side = self._side = torch.cuda.Stream(device=cur.device)
+        final = [t.clone() for t in self._state()]
+        side.wait_stream(cur)  # the forward pass and the final-state copy are done
+        recorded = [None] * len(segments)
+        consumed = [None, None]  # per buffer: the transposed steps that last read it
+
+        def recompute(s):
+            b = s % 2
+            with torch.cuda.stream(side):
+                if consumed[b] is not None:
+                    side.wait_event(consumed[b])
+                for dst, src in zip(self._state(), checkpoints[s]):
+                    dst.copy_(src)
+                q, m = segments[s]
+                adj.run_recorded(q, m, curl[b], plane[b], side.cuda_stream)
+                recorded[s] = torch.cuda.Event()
+                recorded[s].record(side)
+
+        last = len(segments) - 1
+        recompute(last)
+        for s in range(last, -1, -1):
+            if s > 0:
+                recompute(s - 1)
+            b = s % 2
+            cur.wait_event(recorded[s])
+            adj.run_adjoint(segments[s][1], curl[b], plane[b], cur.cuda_stream)
+            consumed[b] = torch.cuda.Event()
+            consumed[b].record(cur)
+        cur.wait_stream(side)
+        for dst, src in zip(self._state(), final):
+            dst.copy_(src)
+        # the checkpoints were read on the side stream
+        for cp in checkpoints:
+            for t in cp:
+                t.record_stream(side)
+
     def _run(self, permittivity, gradient):
         g = self.grid
         perm_b = self._set_permittivity(permittivity)
@@ -224,8 +271,12 @@ class IntensityGradient:
         psi = torch.zeros(_native.Adjoint.psi_size(plan), dtype=dt, device=dev)
         grad = torch.zeros((3, self.nd), dtype=torch.float64, device=dev)
         adj = _native.Adjoint(plan, gcoef, dcoef, lamE, lamH, psi, self.design_box, self.box, grad)
-        curl = torch.empty((K, 3, self.nd), dtype=dt, device=dev)
-        plane = torch.empty((K, 3, self.np), dtype=dt, device=dev)
+        # two record buffers: in the checkpointed mode the recomputation of
+        # segment s-1 fills one while the transposed steps of segment s read
+        # the other
+        nbuf = 2 if gradient and mode == "sqrt" else 1
+        curl = [torch.empty((K, 3, self.nd), dtype=dt, device=dev) for _ in range(nbuf)]
+        plane = [torch.empty((K, 3, self.np), dtype=dt, device=dev) for _ in range(nbuf)]
         J = torch.zeros((), dtype=torch.float64, device=dev)
 
         self._reset()
@@ -235,24 +286,17 @@ class IntensityGradient:
             if gradient and mode == "sqrt":
                 checkpoints.append([t.clone() for t in self._state()])
             # the curl is needed now only when nothing is recomputed later
-            adj.run_recorded(q, m, curl if mode == "record" else None, plane, stream)
-            J += plane[:m].to(torch.float64).square().sum()
+            adj.run_recorded(q, m, curl[0] if mode == "record" else None, plane[0], stream)
+            J += plane[0][:m].to(torch.float64).square().sum()
         g.time_steps_passed = T
         self.last_info = {"mode": mode, "segment": K, "checkpoints": len(checkpoints)}
         if not gradient:
             return float(J), None
-        final = [t.clone() for t in self._state()] if mode == "sqrt" else None
-        for s in range(len(segments) - 1, -1, -1):
-            q, m = segments[s]
-            if mode == "sqrt":
-                for dst, src in zip(self._state(), checkpoints[s]):
-                    dst.copy_(src)
-                checkpoints[s] = None
-                adj.run_recorded(q, m, curl, plane, stream)
-            adj.run_adjoint(m, curl, plane, stream)
-        if final is not None:
-            for dst, src in zip(self._state(), final):
-                dst.copy_(src)
+        if mode == "record":
+            for s in range(len(segments) - 1, -1, -1):
+                adj.run_adjoint(segments[s][1], curl[0], plane[0], stream)
+        else:
+            self._reverse_checkpointed(adj, segments, checkpoints, curl, plane)
         d = self.design
         g_inv = grad.reshape(3, d.Nx, d.Ny, d.Nz).permute(1, 2, 3, 0).clone()
         g_inv[self._border] = 0

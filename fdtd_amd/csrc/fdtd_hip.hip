@@ -109,6 +109,7 @@ struct FdtdPlan {
   int block_threads = 256;   // threads per workgroup of the main kernels
   bool fold_points = false;  // point sources added inside the E kernel
   bool nontemporal = true;   // read-once streams marked non-temporal
+  bool cyclic = false;       // x-chunks dealt cyclically to the XCDs (large grids)
   // FDTD_F32C: 8-bit compensation terms of E and H
   void* E_lo = nullptr;
   void* H_lo = nullptr;
@@ -242,6 +243,7 @@ MainArgs<FS, MS> main_args(const FdtdPlan* p, bool is_H) {
   a.bz_shift = bzs;
   a.cx = p->cx;
   a.by = std::max(1, p->block_threads >> bzs);
+  a.cyclic = p->cyclic;
   a.npts = 0;
   return a;
 }
@@ -252,7 +254,7 @@ void launch_main(const FdtdPlan* p, const MainArgs<FS, MS>& a, bool is_H, hipStr
   const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
   const unsigned nchunks = (unsigned)((p->d.nx + a.cx - 1) / a.cx);
   // x-chunks dealt cyclically to the 8 XCDs (xcd_chunk_tile): whole rounds of 8
-  const dim3 grid(tiles * ((nchunks + 7) / 8 * 8)), block(bz * by);
+  const dim3 grid(tiles * (a.cyclic ? (nchunks + 7) / 8 * 8 : nchunks)), block(bz * by);
   if (a.rec) {  // recording forward pass of a gradient (plain storage)
     if (a.nobjs)
       hipLaunchKernelGGL((k_main_E<FS, MS, O, true, true, true>), grid, block, 0, st, a);
@@ -857,6 +859,9 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     int64_t cx = tiles * p->d.nx / 32768;
     if (const char* env = getenv("FDTD_HIP_CX")) cx = atoi(env);
     if (const char* env = getenv("FDTD_HIP_NT")) p->nontemporal = atoi(env) != 0;
+    // cyclic dealing pays once the fields outgrow the caches (16.7M cells = 256^3)
+    p->cyclic = p->d.cs >= (int64_t)1 << 24;
+    if (const char* env = getenv("FDTD_HIP_CYCLIC")) p->cyclic = atoi(env) != 0;
     p->cx = (int)std::max<int64_t>(1, std::min<int64_t>(cx, p->d.nx));
   }
   // point sources fold into the E kernel when nothing runs between the main

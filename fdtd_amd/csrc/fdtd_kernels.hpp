@@ -321,33 +321,11 @@ __device__ __forceinline__ void pml_face(const FaceDev& fd, const Dims& d, int i
   v2 = n2;
 }
 
-// The faces among the first 64 whose slab meets the block [x0,x1) x [y0,y1)
-// x [z0,z1) of cells, as a bit mask (workgroup-uniform). A tile in the
-// interior meets none and skips the face loop altogether.
-__device__ __forceinline__ uint64_t faces_meeting(const FaceDev* __restrict__ faces, int nfaces, const Dims& d, int x0,
-                                                  int x1, int y0, int y1, int z0, int z1) {
-  uint64_t m = 0;
-  for (int f = 0; f < nfaces && f < 64; ++f) {
-    const FaceDev& fd = faces[f];
-    const int ax = fd.axis;
-    const int n = ax == 0 ? d.nx : (ax == 1 ? d.ny : d.nz);
-    const int lo = ax == 0 ? x0 : (ax == 1 ? y0 : z0), hi = ax == 0 ? x1 : (ax == 1 ? y1 : z1);
-    const int s0 = fd.high ? n - fd.t : 0, s1 = fd.high ? n : fd.t;
-    if (lo < s1 && s0 < hi) m |= 1ull << f;
-  }
-  return m;
-}
-
-// The faces of `mask` (then every face past the 64th) that contain cell
-// (i,j,k), applied in registration order.
+// All faces in `faces` that contain cell (i,j,k), in registration order.
 template <bool IS_E, typename T>
-__device__ __forceinline__ void pml_cell_masked(const FaceDev* __restrict__ faces, uint64_t mask, int nfaces,
-                                                const Dims& d, int i, int j, int k, const Diffs<T>& df, const T kc[3],
-                                                T v[3]) {
-  for (uint64_t m = mask; m; m &= m - 1)
-    pml_face<IS_E>(faces[__builtin_ctzll(m)], d, i, j, k, df.zx, df.yx, df.zy, df.xy, df.yz, df.xz, kc[0], kc[1],
-                   kc[2], v[0], v[1], v[2]);
-  for (int fi = 64; fi < nfaces; ++fi)
+__device__ __forceinline__ void pml_cell(const FaceDev* __restrict__ faces, int nfaces, const Dims& d, int i, int j,
+                                         int k, const Diffs<T>& df, const T kc[3], T v[3]) {
+  for (int fi = 0; fi < nfaces; ++fi)
     pml_face<IS_E>(faces[fi], d, i, j, k, df.zx, df.yx, df.zy, df.xy, df.yz, df.xz, kc[0], kc[1], kc[2], v[0], v[1],
                    v[2]);
 }
@@ -394,14 +372,16 @@ __device__ __forceinline__ unsigned xcd_tile(unsigned w, unsigned nb) {
   return x < rem ? x * (per + 1) + idx : rem * (per + 1) + (x - rem) * per + idx;
 }
 
-// Tile of workgroup w in the single-grid kernels, or -1 for an idle
-// workgroup. The x-chunks (`per_chunk` tiles each) are dealt to the XCDs
-// cyclically -- chunk c to XCD c mod 8 -- and each XCD walks the tiles of its
-// chunks in order, so at any moment the 8 XCDs work on 8 neighbouring chunks:
-// the CPML slabs at both x ends, which cost more per cell, are shared by all
-// XCDs instead of falling to the first and last (workgroups are dispatched in
-// order, so one slow XCD holds back the others). The grid has
-// 8 * ceil(nchunks / 8) * per_chunk workgroups.
+// Tile of workgroup w in the single-grid kernels of large grids, or -1 for
+// an idle workgroup. The x-chunks (`per_chunk` tiles each) are dealt to the
+// XCDs cyclically -- chunk c to XCD c mod 8 -- and each XCD walks the tiles of
+// its chunks in order, so at any moment the 8 XCDs work on 8 neighbouring
+// chunks: the CPML slabs at both x ends, which cost more per cell, are shared
+// by all XCDs instead of falling to the first and last (workgroups are
+// dispatched in order, so one slow XCD holds back the others). The price is
+// the x-halo plane of each chunk, read by another XCD; grids small enough to
+// live in the L2s and the memory-side cache keep the contiguous dealing of
+// xcd_tile. The grid has 8 * ceil(nchunks / 8) * per_chunk workgroups.
 __device__ __forceinline__ int64_t xcd_chunk_tile(unsigned w, unsigned per_chunk, unsigned nchunks) {
   const unsigned x = w % 8, k = w / 8;
   const unsigned tx = (k / per_chunk) * 8 + x;
@@ -449,6 +429,7 @@ struct MainArgs {
   int32_t bz_shift;  // block extent along z is 1 << bz_shift
   int32_t cx;        // planes per workgroup along x
   int32_t by;        // block extent along y (threads per block = bz * by)
+  int32_t cyclic;    // single-grid launch: x-chunks dealt cyclically to the XCDs
   // point sources folded into the E kernel (no periodic stage, no detector):
   // their cells, in registration order, and the waveform table
   int32_t npts;
@@ -470,19 +451,8 @@ struct MainArgs {
 // unconditionally and at once, so a plane costs one memory round trip:
 // threads past the grid's edge read the nearest cell and store nothing, and
 // at the low edges the neighbour offset is 0 (the difference is dropped).
-// FULL: every thread of the tile is inside the grid. The body is then free of
-// lane-dependent branches around the stores, which would otherwise let the
-// compiler sink the plane's loads behind them (two round trips per plane).
-template <class FS, class MS>
-__device__ __forceinline__ bool tile_full(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
-  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
-  const int ntz = (a.d.nz + bz - 1) >> bzs, nty = (a.d.ny + by - 1) / by;
-  const int tz = tile % ntz, ty = (tile / ntz) % nty;
-  return (tz + 1) * bz <= a.d.nz && (ty + 1) * by <= a.d.ny;
-}
-
-template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC, bool FULL>
-__device__ __forceinline__ void main_E_body(const MainArgs<FS, MS>& __restrict__ a, unsigned tile, int64_t q) {
+template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC = false>
+__device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile, int64_t q) {
   using T = typename FS::R;
   const Dims d = a.d;
   const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
@@ -495,9 +465,6 @@ __device__ __forceinline__ void main_E_body(const MainArgs<FS, MS>& __restrict__
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz;
   const int64_t zm = k > 0 ? 1 : 0, ym = j > 0 ? sy : 0;
   const T zero = T(0);
-  const uint64_t fmask = a.nfaces ? faces_meeting(a.faces, a.nfaces, d, i0, i1, ty * by, min(ty * by + by, d.ny),
-                                                  tz * bz, min(tz * bz + bz, d.nz))
-                                  : 0;
   int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
   T hz_prev = zero, hy_prev = zero;
   if (i0 > 0) {
@@ -530,7 +497,7 @@ __device__ __forceinline__ void main_E_body(const MainArgs<FS, MS>& __restrict__
       kc[c] = a.sc * m[c];
       e[c] = e[c] + kc[c] * curl[c];
     }
-    if (FULL || act) {
+    if (act) {
       if (REC) {
         const int64_t r = box_index(a.rb, i, j, k);
         if (r >= 0) {
@@ -540,7 +507,7 @@ __device__ __forceinline__ void main_E_body(const MainArgs<FS, MS>& __restrict__
         }
       }
       if (OBJS) objects_update_E<T, O>(a.objs, a.nobjs, a.sc, i, j, k, curl, e);
-      if (fmask || a.nfaces > 64) pml_cell_masked<true>(a.faces, fmask, a.nfaces, d, i, j, k, df, kc, e);
+      if (a.nfaces) pml_cell<true>(a.faces, a.nfaces, d, i, j, k, df, kc, e);
       for (int s = 0; s < a.npts; ++s)  // sources.py:109, registration order
         if (a.pt_cell[s] == cell) e[2] = e[2] + (T)gp(a.wave)[s * a.wave_cap + q];
       a.F.template put3_s<NT>(cell, e);
@@ -548,12 +515,6 @@ __device__ __forceinline__ void main_E_body(const MainArgs<FS, MS>& __restrict__
     hz_prev = hz;
     hy_prev = hy;
   }
-}
-
-template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC = false>
-__device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile, int64_t q) {
-  if (tile_full(a, tile)) main_E_body<FS, MS, O, OBJS, NT, REC, true>(a, tile, q);
-  else main_E_body<FS, MS, O, OBJS, NT, REC, false>(a, tile, q);
 }
 
 // tiles per x-chunk and the number of x-chunks of a single-grid launch
@@ -564,11 +525,21 @@ __device__ __forceinline__ void chunk_counts(const MainArgs<FS, MS>& a, unsigned
   nchunks = (unsigned)((a.d.nx + a.cx - 1) / a.cx);
 }
 
-template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC>
-__global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
+template <class FS, class MS>
+__device__ __forceinline__ int64_t launch_tile(const MainArgs<FS, MS>& a, unsigned& ntiles) {
   unsigned per, nch;
   chunk_counts(a, per, nch);
-  const int64_t t = xcd_chunk_tile(blockIdx.x, per, nch);
+  ntiles = per * nch;
+  return a.cyclic ? xcd_chunk_tile(blockIdx.x, per, nch) : (int64_t)xcd_tile(blockIdx.x, gridDim.x);
+}
+
+// At least 6 waves per SIMD: the object variants otherwise take ~118 VGPRs
+// (4 waves) for the anisotropic 3x3 path; under this bound they fit in ~76
+// without spilling, and the other variants are below it anyway.
+template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC>
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_main_E(MainArgs<FS, MS> a) {
+  unsigned n;
+  const int64_t t = launch_tile(a, n);
   if (t >= 0) main_E_tile<FS, MS, O, OBJS, NT, REC>(a, (unsigned)t, a.q);
 }
 
@@ -576,8 +547,8 @@ __global__ void __launch_bounds__(1024) k_main_E(MainArgs<FS, MS> a) {
 // forward ones, so workgroups take the tiles in decreasing order (the +y/+z
 // neighbour tiles were read just before, by the same XCD) and march along -x,
 // keeping E of plane i+1 in registers.
-template <class FS, class MS, bool NT, bool FULL>
-__device__ __forceinline__ void main_H_body(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
+template <class FS, class MS, bool NT>
+__device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
   using T = typename FS::R;
   const Dims d = a.d;
   const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
@@ -590,9 +561,6 @@ __device__ __forceinline__ void main_H_body(const MainArgs<FS, MS>& __restrict__
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz;
   const int64_t zp = k < d.nz - 1 ? 1 : 0, yp = j < d.ny - 1 ? sy : 0;
   const T zero = T(0);
-  const uint64_t fmask = a.nfaces ? faces_meeting(a.faces, a.nfaces, d, i0, i1, ty * by, min(ty * by + by, d.ny),
-                                                  tz * bz, min(tz * bz + bz, d.nz))
-                                  : 0;
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   T ez_next = zero, ey_next = zero;  // E at plane i+1
   if (i1 < d.nx) {
@@ -625,9 +593,9 @@ __device__ __forceinline__ void main_H_body(const MainArgs<FS, MS>& __restrict__
       kc[c] = a.sc * m[c];
       h[c] = h[c] - kc[c] * curl[c];
     }
-    if (FULL || act) {
+    if (act) {
       // objects leave H unchanged (objects.py:131-137, 223-229, 271-277)
-      if (fmask || a.nfaces > 64) pml_cell_masked<false>(a.faces, fmask, a.nfaces, d, i, j, k, df, kc, h);
+      if (a.nfaces) pml_cell<false>(a.faces, a.nfaces, d, i, j, k, df, kc, h);
       a.F.template put3_s<NT>(cell, h);
     }
     ez_next = ez;
@@ -636,17 +604,10 @@ __device__ __forceinline__ void main_H_body(const MainArgs<FS, MS>& __restrict__
 }
 
 template <class FS, class MS, bool NT>
-__device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
-  if (tile_full(a, tile)) main_H_body<FS, MS, NT, true>(a, tile);
-  else main_H_body<FS, MS, NT, false>(a, tile);
-}
-
-template <class FS, class MS, bool NT>
 __global__ void __launch_bounds__(1024) k_main_H(MainArgs<FS, MS> a) {
-  unsigned per, nch;
-  chunk_counts(a, per, nch);
-  const int64_t t = xcd_chunk_tile(blockIdx.x, per, nch);
-  if (t >= 0) main_H_tile<FS, MS, NT>(a, (unsigned)((int64_t)per * nch - 1 - t));
+  unsigned n;
+  const int64_t t = launch_tile(a, n);
+  if (t >= 0) main_H_tile<FS, MS, NT>(a, n - 1 - (unsigned)t);
 }
 
 // ------------------------------------------------------------ batched kernels
@@ -669,7 +630,7 @@ struct BatchArgs {
 };
 
 template <class FS, class MS, typename O, bool OBJS, bool NT>
-__global__ void __launch_bounds__(1024) k_batch_E(BatchArgs<FS, MS> b) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_batch_E(BatchArgs<FS, MS> b) {
   const int64_t q = *b.q;
   for (int64_t it = xcd_tile(blockIdx.x, gridDim.x); it < b.total; it += gridDim.x) {
     // the item and its grid are the same for the whole workgroup: say so,
@@ -1066,51 +1027,40 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
   const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
   const unsigned tile = xcd_tile(blockIdx.x, gridDim.x);
   const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
-  const int lz = threadIdx.x & (bz - 1);
-  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  // as in the main kernels: every load of a plane at once, threads past the
+  // edge on the nearest cell (storing nothing), neighbour offset 0 at an edge
+  const int k0 = tz * bz + (threadIdx.x & (bz - 1)), j0 = ty * by + (threadIdx.x >> bzs);
+  const bool act = k0 < d.nz && j0 < d.ny;
+  const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
   const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
-  const bool act = k < d.nz && j < d.ny;
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const int64_t zm = k > 0 ? 1 : 0, ym = j > 0 ? sy : 0;
   const T zero = T(0), sc = a.sc;
   auto Y = [&](int c, int64_t x) { return (sc * a.inv_mu[c * cs + x]) * a.lamH[c * cs + x]; };
   int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
   T yz_prev = zero, yy_prev = zero;  // Y at plane i-1
-  if (act && i0 > 0) {
+  if (i0 > 0) {
     yz_prev = Y(2, cell - sx);
     yy_prev = Y(1, cell - sx);
   }
   for (int i = i0; i < i1; ++i, cell += sx) {
-    T y0 = zero, y1 = zero, y2 = zero;
-    if (act) {
-      y0 = Y(0, cell);
-      y1 = Y(1, cell);
-      y2 = Y(2, cell);
+    const T y0 = Y(0, cell), y1 = Y(1, cell), y2 = Y(2, cell);
+    const T yy_km = Y(1, cell - zm), yx_km = Y(0, cell - zm);
+    const T yz_jm = Y(2, cell - ym), yx_jm = Y(0, cell - ym);
+    T e0 = a.lamE[cell], e1 = a.lamE[cs + cell], e2 = a.lamE[2 * cs + cell];
+    if (a.dcoef) {
+      e0 = a.dcoef[cell] * e0;
+      e1 = a.dcoef[cs + cell] * e1;
+      e2 = a.dcoef[2 * cs + cell] * e2;
     }
-    T yy_km = __shfl_up(y1, 1);
-    T yx_km = __shfl_up(y0, 1);
+    const bool km = k >= 1, kp = k <= d.nz - 2, jm = j >= 1, jp = j <= d.ny - 2, im = i >= 1, ip = i <= d.nx - 2;
+    const T t0 = ((km ? yy_km : zero) - (kp ? y1 : zero)) - ((jm ? yz_jm : zero) - (jp ? y2 : zero));
+    const T t1 = ((im ? yz_prev : zero) - (ip ? y2 : zero)) - ((km ? yx_km : zero) - (kp ? y0 : zero));
+    const T t2 = ((jm ? yx_jm : zero) - (jp ? y0 : zero)) - ((im ? yy_prev : zero) - (ip ? y1 : zero));
+    e0 = e0 - t0;
+    e1 = e1 - t1;
+    e2 = e2 - t2;
     if (act) {
-      if (lz == 0 && k > 0) {
-        yy_km = Y(1, cell - 1);
-        yx_km = Y(0, cell - 1);
-      }
-      T yz_jm = zero, yx_jm = zero;
-      if (j > 0) {
-        yz_jm = Y(2, cell - sy);
-        yx_jm = Y(0, cell - sy);
-      }
-      const bool km = k >= 1, kp = k <= d.nz - 2, jm = j >= 1, jp = j <= d.ny - 2, im = i >= 1, ip = i <= d.nx - 2;
-      const T t0 = ((km ? yy_km : zero) - (kp ? y1 : zero)) - ((jm ? yz_jm : zero) - (jp ? y2 : zero));
-      const T t1 = ((im ? yz_prev : zero) - (ip ? y2 : zero)) - ((km ? yx_km : zero) - (kp ? y0 : zero));
-      const T t2 = ((jm ? yx_jm : zero) - (jp ? y0 : zero)) - ((im ? yy_prev : zero) - (ip ? y1 : zero));
-      T e0 = a.lamE[cell], e1 = a.lamE[cs + cell], e2 = a.lamE[2 * cs + cell];
-      if (a.dcoef) {
-        e0 = a.dcoef[cell] * e0;
-        e1 = a.dcoef[cs + cell] * e1;
-        e2 = a.dcoef[2 * cs + cell] * e2;
-      }
-      e0 = e0 - t0;
-      e1 = e1 - t1;
-      e2 = e2 - t2;
       {
         T e[3] = {e0, e1, e2};
         adj_faces_H<0>(a, i, j, k, y1, y2, yy_prev, yz_prev, e);
@@ -1158,43 +1108,31 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
   const unsigned tile = gridDim.x - 1 - xcd_tile(blockIdx.x, gridDim.x);
   const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
-  const int lz = threadIdx.x & (bz - 1);
-  const int k = tz * bz + lz, j = ty * by + (threadIdx.x >> bzs);
+  const int k0 = tz * bz + (threadIdx.x & (bz - 1)), j0 = ty * by + (threadIdx.x >> bzs);
+  const bool act = k0 < d.nz && j0 < d.ny;
+  const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
   const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
-  const bool act = k < d.nz && j < d.ny;
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const int64_t zp = k < d.nz - 1 ? 1 : 0, yp = j < d.ny - 1 ? sy : 0;
   const T zero = T(0), sc = a.sc;
   auto Y = [&](int c, int64_t x) { return a.gcoef[c * cs + x] * a.lamE[c * cs + x]; };
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   T yz_next = zero, yy_next = zero;  // Y at plane i+1
-  if (act && i1 < d.nx) {
+  if (i1 < d.nx) {
     yz_next = Y(2, cell + sx);
     yy_next = Y(1, cell + sx);
   }
   for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
-    T y0 = zero, y1 = zero, y2 = zero;
+    const T y0 = Y(0, cell), y1 = Y(1, cell), y2 = Y(2, cell);
+    const T yy_kp = Y(1, cell + zp), yx_kp = Y(0, cell + zp);
+    const T yz_jp = Y(2, cell + yp), yx_jp = Y(0, cell + yp);
+    const T l0 = a.lamH[cell], l1 = a.lamH[cs + cell], l2 = a.lamH[2 * cs + cell];
+    const bool km = k >= 1, kp = k <= d.nz - 2, jm = j >= 1, jp = j <= d.ny - 2, im = i >= 1, ip = i <= d.nx - 2;
+    const T t0 = ((km ? y1 : zero) - (kp ? yy_kp : zero)) - ((jm ? y2 : zero) - (jp ? yz_jp : zero));
+    const T t1 = ((im ? y2 : zero) - (ip ? yz_next : zero)) - ((km ? y0 : zero) - (kp ? yx_kp : zero));
+    const T t2 = ((jm ? y0 : zero) - (jp ? yx_jp : zero)) - ((im ? y1 : zero) - (ip ? yy_next : zero));
+    T h0 = l0 + t0, h1 = l1 + t1, h2 = l2 + t2;
     if (act) {
-      y0 = Y(0, cell);
-      y1 = Y(1, cell);
-      y2 = Y(2, cell);
-    }
-    T yy_kp = __shfl_down(y1, 1);
-    T yx_kp = __shfl_down(y0, 1);
-    if (act) {
-      if (lz == bz - 1 && k < d.nz - 1) {
-        yy_kp = Y(1, cell + 1);
-        yx_kp = Y(0, cell + 1);
-      }
-      T yz_jp = zero, yx_jp = zero;
-      if (j < d.ny - 1) {
-        yz_jp = Y(2, cell + sy);
-        yx_jp = Y(0, cell + sy);
-      }
-      const bool km = k >= 1, kp = k <= d.nz - 2, jm = j >= 1, jp = j <= d.ny - 2, im = i >= 1, ip = i <= d.nx - 2;
-      const T t0 = ((km ? y1 : zero) - (kp ? yy_kp : zero)) - ((jm ? y2 : zero) - (jp ? yz_jp : zero));
-      const T t1 = ((im ? y2 : zero) - (ip ? yz_next : zero)) - ((km ? y0 : zero) - (kp ? yx_kp : zero));
-      const T t2 = ((jm ? y0 : zero) - (jp ? yx_jp : zero)) - ((im ? y1 : zero) - (ip ? yy_next : zero));
-      T h0 = a.lamH[cell] + t0, h1 = a.lamH[cs + cell] + t1, h2 = a.lamH[2 * cs + cell] + t2;
       {
         T h[3] = {h0, h1, h2};
         adj_faces_E<0>(a, i, j, k, cell, h);
