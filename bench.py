@@ -123,7 +123,7 @@ def profiled_traffic(kernel, ctype, n, pml):
         return None
     for line in open(TRAFFIC_PROFILE):
         cells = [c.strip().strip("`") for c in line.split("|")]
-        if len(cells) > 5 and cells[1].startswith(f"{kernel}<{ctype}"):
+        if len(cells) > 5 and cells[1] == f"{kernel}<{ctype}>":
             return int(float(cells[4]))
     return None
 
@@ -161,8 +161,8 @@ def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps):
         roofline={"bound": "hbm", "kernel": f"{dom}<{ {'float64': 'double', 'float32': 'float'}.get(dtype, dtype)}>",
                   "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": round(achieved / HBM_PEAK_GBS, 4),
-                  "traffic": (profiled_traffic(dom, "double" if itemsize == 8 else "float", n, pml)
-                              if dtype != "float32c" else None),
+                  "traffic": profiled_traffic(dom, {"float64": "double", "float32": "float"}.get(dtype, "compensated"),
+                                              n, pml),
                   "algorithmic_bytes_per_launch": half_bytes,
                   "avg_launch_ms": {k: round(v, 5) for k, v in per_launch.items()}})
 
