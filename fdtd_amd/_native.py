@@ -11,7 +11,7 @@ import os
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
 
-F64, F32, F32C = 0, 1, 2
+F64, F32, F32C, C128, C64 = 0, 1, 2, 3, 4
 OBJ_PLAIN, OBJ_ABSORBING, OBJ_ANISOTROPIC = 0, 1, 2
 SRC_POINT, SRC_LINE, SRC_PLANE = 0, 1, 2
 

@@ -73,8 +73,8 @@ class IntensityGradient:
     """
 
     def __init__(self, grid, design, box, steps, checkpoint="sqrt"):
-        if grid.compensated:
-            raise NotImplementedError("gradients need hip.float64 or hip.float32 storage")
+        if grid.compensated or grid.complex:
+            raise NotImplementedError("gradients need real hip.float64 or hip.float32 fields")
         if type(design) is not Object or design.grid is not grid:
             raise ValueError("the design region must be a plain Object registered in the grid")
         if grid.boundaries and any(not isinstance(b, PML) for b in grid.boundaries):

@@ -53,6 +53,11 @@ class Boundary:
             return s
         raise ValueError("Invalid grid indexing used for boundary")
 
+    def promote_dtypes_to_complex(self):
+        # boundaries.py:106-107: the base class converts CPML state, which only
+        # a PML has; anything else fails on its first attribute
+        raise AttributeError(f"'{type(self).__name__}' object has no attribute 'phi_E'")
+
     # plan interface
     def _plan_key(self):
         return (type(self).__name__,)
@@ -202,7 +207,7 @@ class PML(Boundary):
 
     # plan interface
     def _plan_key(self):
-        return (type(self).__name__, self.thickness, self.a)
+        return (type(self).__name__, self.thickness, self.a, id(self._psiE))
 
     def _add_to_plan(self, plan):
         if self.axis is None:
@@ -266,7 +271,10 @@ class PML(Boundary):
     phi_H = property(lambda self: self._phi(self._psiH))
 
     def promote_dtypes_to_complex(self):
-        raise NotImplementedError("complex fields are not supported by the HIP backend")
+        """The CPML state becomes complex (boundaries.py:106-116)."""
+        cdt = self.grid.field_dtype
+        self._psiE = self._psiE.to(cdt)
+        self._psiH = self._psiH.to(cdt)
 
     def __repr__(self):
         return f"PML(name={repr(self.name)})"

@@ -166,6 +166,8 @@ int check_plan(const FdtdPlan* p) {
 
 int cell_ok(const FdtdPlan* p, int64_t c) { return c >= 0 && c < p->d.cs; }
 
+bool is_complex(const FdtdPlan* p) { return p->dtype == FDTD_C128 || p->dtype == FDTD_C64; }
+
 // log2 of the lanes of a z-row of nz cells, one cell per lane (at most a
 // wavefront)
 int lanes_shift(int nz) {
@@ -208,8 +210,24 @@ template <>
 CompField make_field(const FdtdPlan* p, bool H) {
   return {reinterpret_cast<float*>(H ? p->H : p->E), reinterpret_cast<float*>(H ? p->H_lo : p->E_lo), p->d.cs};
 }
+template <>
+PlainField<cplx<double>, cplx<double>> make_field(const FdtdPlan* p, bool H) {
+  return {reinterpret_cast<cplx<double>*>(H ? p->H : p->E), p->d.cs};
+}
+template <>
+PlainField<cplx<float>, cplx<float>> make_field(const FdtdPlan* p, bool H) {
+  return {reinterpret_cast<cplx<float>*>(H ? p->H : p->E), p->d.cs};
+}
 template <class MS>
 MS make_mat(const FdtdPlan* p, bool mu);
+template <>
+MatArray<double, cplx<double>> make_mat(const FdtdPlan* p, bool mu) {
+  return {reinterpret_cast<const double*>(mu ? p->inv_mu : p->inv_eps), p->d.cs};
+}
+template <>
+MatArray<float, cplx<float>> make_mat(const FdtdPlan* p, bool mu) {
+  return {reinterpret_cast<const float*>(mu ? p->inv_mu : p->inv_eps), p->d.cs};
+}
 template <>
 MatArray<double, double> make_mat(const FdtdPlan* p, bool mu) {
   return {reinterpret_cast<const double*>(mu ? p->inv_mu : p->inv_eps), p->d.cs};
@@ -357,6 +375,11 @@ int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
 template <class Fn>
 int with_policies(const FdtdPlan* p, Fn&& fn) {
   const bool pal = p->eps_id != nullptr;
+  // complex fields: real grid materials, complex object arrays, no palette
+  if (p->dtype == FDTD_C128)
+    return fn.template operator()<PlainField<cplx<double>, cplx<double>>, MatArray<double, cplx<double>>, cplx<double>>();
+  if (p->dtype == FDTD_C64)
+    return fn.template operator()<PlainField<cplx<float>, cplx<float>>, MatArray<float, cplx<float>>, cplx<float>>();
   if (p->dtype == FDTD_F64)
     return pal ? fn.template operator()<PlainField<double, double>, MatPalette<double>, double>()
                : fn.template operator()<PlainField<double, double>, MatArray<double, double>, double>();
@@ -489,6 +512,7 @@ int recorded_steps(FdtdAdjoint* a, int64_t q0, int64_t nsteps, T* curl, T* plane
 int batch_check_plan(const FdtdPlan* p, const FdtdPlan* p0) {
   if (int rc = check_plan(p)) return rc;
   if (p->dtype != p0->dtype) return fail(-1, "batched grids must share one storage type");
+  if (is_complex(p)) return fail(-1, "batched grids cannot have complex fields");
   if ((p->eps_id != nullptr) != (p0->eps_id != nullptr)) return fail(-1, "batched grids must all use a palette or none");
   if (!p->stages.empty()) return fail(-1, "batched grids cannot have periodic boundaries");
   if (!p->dets.empty()) return fail(-1, "batched grids cannot have detectors");
@@ -616,7 +640,8 @@ int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number, v
                      const void* inv_eps, const void* inv_mu, FdtdPlan** out) {
   if (!out) return fail(-1, "null output pointer");
   *out = nullptr;
-  if (dtype != FDTD_F64 && dtype != FDTD_F32 && dtype != FDTD_F32C) return fail(-1, "unknown dtype");
+  if (dtype != FDTD_F64 && dtype != FDTD_F32 && dtype != FDTD_F32C && dtype != FDTD_C128 && dtype != FDTD_C64)
+    return fail(-1, "unknown dtype");
   if (nx < 1 || ny < 1 || nz < 1) return fail(-1, "grid dimensions must be positive");
   if (nx > 65535) return fail(-1, "nx above 65535 is not supported");
   if (!E || !H || !inv_eps || !inv_mu) return fail(-1, "null field or material pointer");
@@ -661,6 +686,7 @@ int fdtd_plan_set_material_palette(FdtdPlan* p, const uint8_t* eps_id, const dou
   if (!p) return fail(-1, "null plan");
   const bool any = eps_id || eps_table || mu_id || mu_table;
   if (any && !(eps_id && eps_table && mu_id && mu_table)) return fail(-1, "palette needs ids and tables for both");
+  if (any && is_complex(p)) return fail(-1, "no material palette on the complex-field path");
   p->eps_id = eps_id;
   p->eps_tab = eps_table;
   p->mu_id = mu_id;
@@ -709,6 +735,8 @@ int fdtd_plan_add_object(FdtdPlan* p, int kind, int x0, int x1, int y0, int y1, 
     return fail(-1, "object box outside the grid or empty");
   if (!inv) return fail(-1, "null object permittivity");
   if (kind == FDTD_OBJ_ABSORBING && !absorption) return fail(-1, "null absorption factor");
+  if (kind != FDTD_OBJ_PLAIN && is_complex(p))
+    return fail(-1, "absorbing and anisotropic objects are not supported on the complex-field path");
   p->objs.push_back({kind, x0, x1, y0, y1, z0, z1, inv, absorption});
   p->finalized = false;
   return 0;
@@ -740,6 +768,7 @@ int fdtd_plan_add_current_detector(FdtdPlan* p, int n, const int32_t* px, const 
                                    double spacing, void* bufI, int64_t capacity, int64_t count) {
   if (!p) return fail(-1, "null plan");
   if (n < 1 || !px || !py || !pz || !bufI) return fail(-1, "current detector without points or buffer");
+  if (is_complex(p)) return fail(-1, "current detectors are not supported on the complex-field path");
   if (count < 0 || count > capacity) return fail(-1, "bad current detector count");
   CurHost c;
   c.pts.resize(3 * (size_t)n);
@@ -766,6 +795,7 @@ int fdtd_plan_add_soft_source(FdtdPlan* p, int64_t cell, double impedance, doubl
   if (!cell_ok(p, cell)) return fail(-1, "source cell outside the grid");
   if (!vin || !vout || count < 0 || count > capacity) return fail(-1, "bad soft source buffers");
   if (detector < 0) return fail(-1, "bad current detector index");
+  if (is_complex(p)) return fail(-1, "soft sources are not supported on the complex-field path");
   SrcHost s;
   s.kind = FDTD_SRC_SOFT;
   s.compE = 2;

@@ -14,6 +14,8 @@
 //
 // The kernels are written once over two policies:
 //   field storage  PlainField<S, R>  values stored as S, computed in R
+//                                    (S = R = cplx<double/float> for the
+//                                    complex-field path)
 //                  CompField         float plus an 8-bit compensation term
 //                                    (the next 8 mantissa bits: a 40-bit
 //                                    format), computed in double
@@ -121,6 +123,52 @@ __device__ __forceinline__ void st_stream(T* p, T v) {
   else *gp(p) = v;
 }
 
+// ------------------------------------------------------------ complex values
+// The reference promotes E, H and the CPML state to complex when an object
+// with a complex permittivity is registered (grid.py:353-356,
+// objects.py:51-52) and then steps with numpy's complex arithmetic. numpy's
+// vectorised loop multiplies complex numbers as
+//   re = fma(ar, br, -(ai * bi)),  im = fma(ar, bi, ai * br)
+// (checked bit for bit against its results), and promotes a real operand x
+// to (x, 0), which makes every real-times-complex product exactly the
+// componentwise one; sums and differences are componentwise. cplx<S> does
+// the same, so the complex path matches the reference bit for bit as well.
+// Division (Smith's method) exists only so that every kernel instantiates;
+// the host refuses the components that would divide complex values.
+template <typename S>
+struct cplx {
+  S re, im;
+  __host__ __device__ cplx() : re(0), im(0) {}
+  __host__ __device__ cplx(S r) : re(r), im(0) {}
+  __host__ __device__ cplx(S r, S i) : re(r), im(i) {}
+  friend __host__ __device__ cplx operator+(cplx a, cplx b) { return cplx(a.re + b.re, a.im + b.im); }
+  friend __host__ __device__ cplx operator-(cplx a, cplx b) { return cplx(a.re - b.re, a.im - b.im); }
+  friend __host__ __device__ cplx operator-(cplx a) { return cplx(-a.re, -a.im); }
+  friend __host__ __device__ cplx operator*(cplx a, cplx b) {
+    return cplx(fma(a.re, b.re, -(a.im * b.im)), fma(a.re, b.im, a.im * b.re));
+  }
+  friend __host__ __device__ cplx operator/(cplx a, cplx b) {
+    if (fabs(b.re) >= fabs(b.im)) {
+      const S rat = b.im / b.re, scl = S(1) / (b.re + b.im * rat);
+      return cplx((a.re + a.im * rat) * scl, (a.im - a.re * rat) * scl);
+    }
+    const S rat = b.re / b.im, scl = S(1) / (b.im + b.re * rat);
+    return cplx((a.re * rat + a.im) * scl, (a.im * rat - a.re) * scl);
+  }
+};
+
+template <bool NT, typename S>
+__device__ __forceinline__ cplx<S> ld_stream(const cplx<S>* p) {
+  const S* q = reinterpret_cast<const S*>(p);
+  return cplx<S>(ld_stream<NT>(q), ld_stream<NT>(q + 1));
+}
+template <bool NT, typename S>
+__device__ __forceinline__ void st_stream(cplx<S>* p, cplx<S> v) {
+  S* q = reinterpret_cast<S*>(p);
+  st_stream<NT>(q, v.re);
+  st_stream<NT>(q + 1, v.im);
+}
+
 // ------------------------------------------------------------ field storage
 // Element i of a field is component i / cs of cell i % cs (component-major).
 // get/put/copy/sample take that element index; get_c/get3/put3 take a cell
@@ -132,7 +180,7 @@ struct PlainField {
   S* p;
   int64_t cs;
   template <int C>
-  __device__ __forceinline__ R get_c(int64_t cell) const { return (R)gp(p)[C * cs + cell]; }
+  __device__ __forceinline__ R get_c(int64_t cell) const { return (R)ld_stream<false>(p + C * cs + cell); }
   template <bool NT>
   __device__ __forceinline__ void get3_s(int64_t cell, R v[3]) const {
 #pragma unroll
@@ -143,14 +191,14 @@ struct PlainField {
 #pragma unroll
     for (int c = 0; c < 3; ++c) st_stream<NT>(p + c * cs + cell, (S)v[c]);
   }
-  __device__ __forceinline__ R get(int64_t i) const { return (R)gp(p)[i]; }
+  __device__ __forceinline__ R get(int64_t i) const { return (R)ld_stream<false>(p + i); }
   template <bool NT>
   __device__ __forceinline__ R get_s(int64_t i) const { return (R)ld_stream<NT>(p + i); }
   template <bool NT>
   __device__ __forceinline__ void put_s(int64_t i, R v) const { st_stream<NT>(p + i, (S)v); }
-  __device__ __forceinline__ void put(int64_t i, R v) const { gp(p)[i] = (S)v; }
-  __device__ __forceinline__ void copy(int64_t dst, int64_t src) const { gp(p)[dst] = gp(p)[src]; }
-  __device__ __forceinline__ Sample sample(int64_t i) const { return gp(p)[i]; }
+  __device__ __forceinline__ void put(int64_t i, R v) const { st_stream<false>(p + i, (S)v); }
+  __device__ __forceinline__ void copy(int64_t dst, int64_t src) const { st_stream<false>(p + dst, ld_stream<false>(p + src)); }
+  __device__ __forceinline__ Sample sample(int64_t i) const { return ld_stream<false>(p + i); }
 };
 
 // Compensated storage: a value v is kept as the float h holding the top 24
@@ -289,20 +337,20 @@ __device__ __forceinline__ void pml_face(const FaceDev& fd, const Dims& d, int i
   const int64_t snz = ax == 2 ? fd.t : d.nz;
   const int64_t li = ax == 0 ? l : i, lj = ax == 1 ? l : j, lk = ax == 2 ? l : k;
   const int64_t off = (li * sny + lj) * snz + lk;
-  const gptr<T> psi = gp(reinterpret_cast<T*>(IS_E ? fd.psiE : fd.psiH));
+  T* const psi = reinterpret_cast<T*>(IS_E ? fd.psiE : fd.psiH);
   const gptr<const double> bt = gp(IS_E ? fd.bE : fd.bH);
   const gptr<const double> ct = gp(IS_E ? fd.cE : fd.cH);
   const T b = (T)bt[l], c = (T)ct[l];
   const T d0 = ax == 0 ? zx : (ax == 1 ? zy : yz);
   const T d1 = ax == 0 ? yx : (ax == 1 ? xy : xz);
-  T p0 = psi[off] * b;
-  T p1 = psi[fd.slab + off] * b;
+  T p0 = ld_stream<false>(psi + off) * b;
+  T p1 = ld_stream<false>(psi + fd.slab + off) * b;
   if (IS_E ? (l >= 1) : (l <= fd.t - 2)) {
     p0 = p0 + d0 * c;
     p1 = p1 + d1 * c;
   }
-  psi[off] = p0;
-  psi[fd.slab + off] = p1;
+  st_stream<false>(psi + off, p0);
+  st_stream<false>(psi + fd.slab + off, p1);
   const T r0 = (ax == 0 ? k1 : k0) * p0;
   const T r1 = (ax == 2 ? k1 : k2) * p1;
   T n0, n1, n2;
@@ -339,25 +387,26 @@ __device__ __forceinline__ void objects_update_E(const ObjDev* __restrict__ objs
     const ObjDev& ob = objs[o];
     if (i < ob.x0 || i >= ob.x1 || j < ob.y0 || j >= ob.y1 || k < ob.z0 || k >= ob.z1) continue;
     const int64_t lo = ((int64_t)(i - ob.x0) * (ob.y1 - ob.y0) + (j - ob.y0)) * (ob.z1 - ob.z0) + (k - ob.z0);
-    const gptr<const O> inv = gp(reinterpret_cast<const O*>(ob.inv));
+    const O* const inv = reinterpret_cast<const O*>(ob.inv);
+    auto at = [](const O* q, int64_t x) { return (T)ld_stream<false>(q + x); };
     if (ob.kind == FDTD_OBJ_PLAIN) {
 #pragma unroll
-      for (int c = 0; c < 3; ++c) e[c] = e[c] + (sc * (T)inv[lo * 3 + c]) * curl[c];
+      for (int c = 0; c < 3; ++c) e[c] = e[c] + (sc * at(inv, lo * 3 + c)) * curl[c];
     } else if (ob.kind == FDTD_OBJ_ABSORBING) {
-      const gptr<const O> fa = gp(reinterpret_cast<const O*>(ob.absorb));
+      const O* const fa = reinterpret_cast<const O*>(ob.absorb);
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
-        const T f = (T)fa[lo * 3 + c];
+        const T f = at(fa, lo * 3 + c);
         e[c] = e[c] * ((T(1) - f) / (T(1) + f));
-        e[c] = e[c] + ((sc * (T)inv[lo * 3 + c]) * curl[c]) / (T(1) + f);
+        e[c] = e[c] + ((sc * at(inv, lo * 3 + c)) * curl[c]) / (T(1) + f);
       }
     } else {
-      const gptr<const O> m = inv + lo * 9;
+      const O* const m = inv + lo * 9;
       T v[3];
 #pragma unroll
       // the association numpy's einsum uses for this 3-term contraction
       for (int r = 0; r < 3; ++r)
-        v[r] = ((T)m[3 * r] * curl[0] + (T)m[3 * r + 2] * curl[2]) + (T)m[3 * r + 1] * curl[1];
+        v[r] = (at(m, 3 * r) * curl[0] + at(m, 3 * r + 2) * curl[2]) + at(m, 3 * r + 1) * curl[1];
 #pragma unroll
       for (int r = 0; r < 3; ++r) e[r] = e[r] + sc * v[r];
     }
@@ -775,14 +824,15 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
       // current its detector measured after the last H half step
       if (tid == 0) {
         const CurDev& cd = a.curs[src.det];
-        const T cur = a.q > 0 ? (T)gp(reinterpret_cast<const Sample*>(cd.buf))[(a.counter_H - 1 - cd.slot0) * cd.n]
+        const T cur = a.q > 0 ? (T)ld_stream<false>(reinterpret_cast<const Sample*>(cd.buf) +
+                                                    (a.counter_H - 1 - cd.slot0) * cd.n)
                               : T(0);
         const T u = src.amplitude > 0 ? w + (T)src.amplitude * cur : w;
         const int64_t cell = base + gp(src.cells)[0];
         a.F.put(cell, a.F.get(cell) + u / (T)src.spacing);
         const int64_t slot = a.counter - src.slot0;
-        gp(reinterpret_cast<Sample*>(src.vin))[slot] = (Sample)w;
-        gp(reinterpret_cast<Sample*>(src.vout))[slot] = (Sample)u;
+        st_stream<false>(reinterpret_cast<Sample*>(src.vin) + slot, (Sample)w);
+        st_stream<false>(reinterpret_cast<Sample*>(src.vout) + slot, (Sample)u);
       }
       __syncthreads();
       continue;
@@ -802,7 +852,7 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
     // detectors.py:388-460, X = 0, Y = 1; numpy wraps index -1 to the last
     for (int t = 0; t < a.ncur; ++t) {
       const CurDev& cd = a.curs[t];
-      const gptr<Sample> buf = gp(reinterpret_cast<Sample*>(cd.buf)) + (a.counter - cd.slot0) * cd.n;
+      Sample* const buf = reinterpret_cast<Sample*>(cd.buf) + (a.counter - cd.slot0) * cd.n;
       const T dx = (T)cd.spacing;
       for (int p = tid; p < cd.n; p += blockDim.x) {
         const int px = gp(cd.pts)[p], py = gp(cd.pts)[cd.n + p], pz = gp(cd.pts)[2 * cd.n + p];
@@ -818,7 +868,7 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
         v1 = (H(0, px, py - 1, pz - 1) - H(0, px, py, pz - 1)) * dx;
         v2 = v2 + (H(1, px, py, pz - 1) - H(1, px - 1, py, pz - 1)) * dx;
         const T c2 = v1 + v2;
-        buf[p] = (Sample)((c1 + c2) / T(2));
+        st_stream<false>(buf + p, (Sample)((c1 + c2) / T(2)));
       }
     }
   }
@@ -832,10 +882,10 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
   for (int t = 0; t < a.ndet; ++t) {
     const DetDev& det = a.dets[t];
     const int64_t slot = a.counter - (a.is_H ? det.slot0H : det.slot0E);
-    const gptr<Sample> buf = gp(reinterpret_cast<Sample*>(a.is_H ? det.bufH : det.bufE)) + slot * det.n * 3;
+    Sample* const buf = reinterpret_cast<Sample*>(a.is_H ? det.bufH : det.bufE) + slot * det.n * 3;
     for (int p = tid; p < det.n * 3; p += blockDim.x) {
       const int c = p % 3;
-      buf[p] = a.F.sample(c * a.d.cs + gp(det.cells)[p / 3]);
+      st_stream<false>(buf + p, a.F.sample(c * a.d.cs + gp(det.cells)[p / 3]));
     }
   }
 }

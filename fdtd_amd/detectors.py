@@ -46,13 +46,18 @@ class _DetectorBase:
         new_cap = max(need, 2 * cap, 64)
         n = len(self._cells)
         g = self.grid
-        bufE = torch.zeros((new_cap, n, 3), dtype=g.dtype, device=g._E.device)
-        bufH = torch.zeros((new_cap, n, 3), dtype=g.dtype, device=g._E.device)
+        bufE = torch.zeros((new_cap, n, 3), dtype=g.field_dtype, device=g._E.device)
+        bufH = torch.zeros((new_cap, n, 3), dtype=g.field_dtype, device=g._E.device)
         if self._bufE is not None:
             bufE[: self._count_E] = self._bufE[: self._count_E]
             bufH[: self._count_H] = self._bufH[: self._count_H]
         self._bufE, self._bufH = bufE, bufH
         return True
+
+    def _promote(self, cdt):
+        """Complex traces once the fields are complex (samples so far kept)."""
+        if self._bufE is not None:
+            self._bufE, self._bufH = self._bufE.to(cdt), self._bufH.to(cdt)
 
     def _plan_key(self):
         return (type(self).__name__, id(self._bufE))
@@ -178,6 +183,9 @@ class CurrentDetector:
     def _points(self):
         px, py, pz = self._pts
         return [(a, b, c) for a in px for b in py for c in pz]
+
+    def _promote(self, cdt):
+        raise NotImplementedError("CurrentDetector is not supported on the complex-field path")
 
     def _reserve(self, extra):
         need = self._count_H + int(extra)

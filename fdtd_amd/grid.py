@@ -83,6 +83,7 @@ class Grid:
         self.dtype = bd.float                  # fields
         self.mat_dtype = bd.material_float     # materials, object arrays, CPML state
         self.compensated = bool(bd.compensated)
+        self.complex = False  # promote_dtypes_to_complex (grid.py:353-356)
         dev = bd.device
         soa = (3, self.Nx, self.Ny, self.Nz)
         self._E = torch.zeros(soa, dtype=self.dtype, device=dev)
@@ -176,6 +177,8 @@ class Grid:
         compensation bits, inserted below the float's mantissa) that E/H --
         the float part alone -- round away; elsewhere it is a cast."""
         soa = self._E if field == "E" else self._H
+        if self.complex:
+            return _soa_view(soa.to(torch.complex128))
         if not self.compensated:
             return _soa_view(soa.to(torch.float64))
         word = (self._E_lo if field == "E" else self._H_lo).to(torch.int64)
@@ -271,8 +274,47 @@ class Grid:
         return list(self.boundaries) + list(self.objects) + list(self.sources) + list(self.detectors)
 
     def _palette_wanted(self):
+        if self.complex:
+            return False
         mode = self.material_palette
         return bool(self.compensated) if mode == "auto" else bool(mode)
+
+    # ------------------------------------------------------ complex fields
+    @property
+    def field_dtype(self):
+        """dtype of E, H, the CPML state and detector traces."""
+        return self._E.dtype
+
+    @property
+    def obj_dtype(self):
+        """dtype of the object arrays: the material dtype, complex on the
+        complex-field path."""
+        if not self.complex:
+            return self.mat_dtype
+        return torch.complex128 if self.mat_dtype == torch.float64 else torch.complex64
+
+    def promote_dtypes_to_complex(self):
+        """E and H become complex, and so does the CPML state of every boundary
+        registered so far (grid.py:353-356); called when an Object with a
+        complex permittivity is registered. A PeriodicBoundary has no such
+        state and fails as in the reference. Complex fields are stepped by the
+        complex128 (hip.float64) or complex64 (hip.float32) kernels, bit for
+        bit against numpy's complex arithmetic in float64."""
+        if self.compensated:
+            raise NotImplementedError("complex fields are not supported by hip.float32c")
+        if self.complex:
+            return
+        cdt = torch.complex128 if self.dtype == torch.float64 else torch.complex64
+        self._E = self._E.to(cdt)
+        self._H = self._H.to(cdt)
+        self.complex = True
+        for obj in self.objects:
+            if obj._kind != _native.OBJ_PLAIN:
+                raise NotImplementedError(f"{type(obj).__name__} is not supported on the complex-field path")
+        for det in self.detectors:
+            det._promote(cdt)
+        for boundary in self.boundaries:
+            boundary.promote_dtypes_to_complex()
 
     def _material_palette(self):
         """(eps_id, eps_table, mu_id, mu_table) or None when a grid material has
@@ -297,7 +339,7 @@ class Grid:
 
     def _current_plan_key(self):
         key = [len(self.boundaries), len(self.objects), len(self.sources), len(self.detectors), self.courant_number,
-               self._palette_wanted()]
+               self._palette_wanted(), self._E.data_ptr(), self._H.data_ptr()]
         if self._palette_wanted():
             key += [self._inv_eps._version, self._inv_mu._version]
         for comp in self._components():
@@ -311,7 +353,14 @@ class Grid:
         key = self._current_plan_key()
         if self._plan is not None and key == self._plan_key:
             return self._plan
-        if self.compensated:
+        if self.complex:
+            for b in self.boundaries:
+                if hasattr(b, "_psiE") and not b._psiE.is_complex():
+                    # the reference's in-place CPML update fails the same way
+                    raise TypeError(f"Cannot cast ufunc 'add' output from dtype('{self.field_dtype}') to the real "
+                                    f"CPML state of {b!r}, registered after the fields became complex")
+            dtype_code = _native.C128 if self.dtype == torch.float64 else _native.C64
+        elif self.compensated:
             dtype_code = _native.F32C
         else:
             dtype_code = _native.F64 if self.dtype == torch.float64 else _native.F32
@@ -447,9 +496,6 @@ class Grid:
         raise NotImplementedError("register components with grid[x, y, z] = component")
 
     add_boundary = add_detector = add_object = add_source
-
-    def promote_dtypes_to_complex(self):
-        raise NotImplementedError("complex fields are not supported by the HIP backend")
 
     def __setitem__(self, key, attr):
         if not isinstance(key, tuple):

@@ -24,6 +24,12 @@ from .backend import backend as bd
 from .typing_ import ListOrSlice, Tensorlike
 
 
+def _is_complex(a):
+    if torch.is_tensor(a):
+        return a.is_complex()
+    return np.iscomplexobj(np.asarray(a))
+
+
 def _host(a, dtype=None):
     if torch.is_tensor(a):
         a = a.detach().cpu().numpy()
@@ -44,8 +50,13 @@ class Object:
     def _register_grid(self, grid, x: ListOrSlice, y: ListOrSlice, z: ListOrSlice):
         self.grid = grid
         self.grid.objects.append(self)
-        if bd.is_complex(self.permittivity):
-            raise NotImplementedError("complex permittivity is not supported by the HIP backend")
+        complex_perm = _is_complex(self.permittivity)
+        if self._kind != _native.OBJ_PLAIN and (complex_perm or grid.complex):
+            raise NotImplementedError(f"{type(self).__name__} is not supported on the complex-field path")
+        if complex_perm:
+            # objects.py:51-52: E, H and the CPML state registered so far
+            # become complex before the object computes its inverse
+            self.grid.promote_dtypes_to_complex()
         if self.name is not None:
             if not hasattr(grid, self.name):
                 setattr(grid, self.name, self)
@@ -59,6 +70,8 @@ class Object:
         self.Nz = abs(self.z.stop - self.z.start)
 
         npdt = bd.numpy_float
+        if complex_perm:  # numpy's complex division, in the fields' complex dtype
+            npdt = np.complex128 if npdt == np.float64 else np.complex64
         perm = _host(self.permittivity, npdt)
         if perm.ndim == 3:
             perm = perm[..., None]
@@ -100,10 +113,10 @@ class Object:
         """Bind `attr` as a contiguous device tensor of the field dtype."""
         v = getattr(self, attr)
         g = self.grid
-        ok = (torch.is_tensor(v) and v.device == g._E.device and v.dtype == g.mat_dtype and v.is_contiguous()
+        ok = (torch.is_tensor(v) and v.device == g._E.device and v.dtype == g.obj_dtype and v.is_contiguous()
               and tuple(v.shape) == shape)
         if not ok:
-            t = torch.as_tensor(_host(v) if not torch.is_tensor(v) else v).to(device=g._E.device, dtype=g.mat_dtype)
+            t = torch.as_tensor(_host(v) if not torch.is_tensor(v) else v).to(device=g._E.device, dtype=g.obj_dtype)
             t = t.expand(shape).contiguous()
             setattr(self, attr, t)
             v = t

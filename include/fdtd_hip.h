@@ -20,6 +20,13 @@
  *     FDTD_F32C: E and H stored as float32 plus an 8-bit compensation term per
  *     value (fdtd_plan_set_compensation), arithmetic in float64, materials,
  *     object arrays and CPML state in float64, detector traces in float32.
+ *   - FDTD_C128 / FDTD_C64: the complex-field path of the reference
+ *     (fdtd/grid.py:353-356): E, H, CPML state and object inverse
+ *     permittivities complex (interleaved re, im; complex128 / complex64),
+ *     grid materials real (float64 / float32), detector traces complex.
+ *     Absorbing and anisotropic objects, soft sources and current detectors
+ *     are refused on this path; so are the material palette, batching and
+ *     the adjoint.
  *   - Every function returns 0 on success and a negative code on error;
  *     fdtd_last_error() then describes the failure. No C++ exception crosses
  *     the ABI.
@@ -36,7 +43,7 @@ extern "C" {
 
 typedef struct FdtdPlan FdtdPlan;
 
-enum { FDTD_F64 = 0, FDTD_F32 = 1, FDTD_F32C = 2 };
+enum { FDTD_F64 = 0, FDTD_F32 = 1, FDTD_F32C = 2, FDTD_C128 = 3, FDTD_C64 = 4 };
 enum { FDTD_OBJ_PLAIN = 0, FDTD_OBJ_ABSORBING = 1, FDTD_OBJ_ANISOTROPIC = 2 };
 enum { FDTD_SRC_POINT = 0, FDTD_SRC_LINE = 1, FDTD_SRC_PLANE = 2, FDTD_SRC_SOFT = 3 };
 

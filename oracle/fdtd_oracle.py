@@ -156,6 +156,12 @@ class _Pml:
         for i in range(3):
             self.phi_H[..., i] = self.psi_H[i][..., (i + 1) % 3] - self.psi_H[i][..., (i + 2) % 3]
 
+    def promote_dtypes_to_complex(self):  # boundaries.py:106-116
+        self.phi_E = self.phi_E.astype(np.complex128)
+        self.phi_H = self.phi_H.astype(np.complex128)
+        self.psi_E = [p.astype(np.complex128) for p in self.psi_E]
+        self.psi_H = [p.astype(np.complex128) for p in self.psi_H]
+
     def update_E(self):
         s = self.sim
         s.E[self.loc] += s.courant_number * s.inverse_permittivity[self.loc] * self.phi_E
@@ -198,7 +204,14 @@ class _Object:
         self.sim, self.kind = sim, kind
         self.x, self.y, self.z = x, y, z
         self.Nx, self.Ny, self.Nz = x.stop - x.start, y.stop - y.start, z.stop - z.start
-        perm = np.array(permittivity, dtype=np.float64)
+        perm = np.array(permittivity)
+        if np.iscomplexobj(perm):
+            # objects.py:51-52: a complex permittivity promotes the grid fields
+            # and the boundaries registered so far before anything else
+            perm = perm.astype(np.complex128)
+            sim.promote_dtypes_to_complex()
+        else:
+            perm = perm.astype(np.float64)
         if perm.ndim == 3:
             perm = perm[..., None]
         inv = np.ones((self.Nx, self.Ny, self.Nz, 3), dtype=perm.dtype) / perm
@@ -400,6 +413,18 @@ class Sim:
         self.time_steps_passed = 0
         self.boundaries, self.objects, self.sources, self.detectors = [], [], [], []
         self._faces_taken = set()
+
+    def promote_dtypes_to_complex(self):
+        """grid.py:353-356: E and H become complex, and so does the CPML state
+        of every boundary registered so far. A periodic boundary has no such
+        state: the reference's Boundary.promote_dtypes_to_complex then fails
+        with AttributeError (boundaries.py:106-107), and so does this."""
+        self.E = self.E.astype(np.complex128)
+        self.H = self.H.astype(np.complex128)
+        for b in self.boundaries:
+            if not hasattr(b, "promote_dtypes_to_complex"):
+                raise AttributeError(f"'{type(b).__name__}' object has no attribute 'phi_E'")
+            b.promote_dtypes_to_complex()
 
     # -- unit handling, grid.py:171-223
     def cells_of(self, d):

@@ -262,9 +262,45 @@ def scenario_soft_source():
     )
 
 
+def scenario_complex_object():
+    """Complex fields (grid.py:353-356, objects.py:51-52, boundaries.py:106-116):
+    an Object with a complex permittivity promotes E, H and the CPML state
+    registered before it to complex128; a second Object carries a per-cell
+    complex permittivity array, a third a real one. PML on five faces
+    (registered first), a pulsed point source, a line source, a line detector
+    and random real initial fields."""
+    rs = np.random.RandomState(71)
+    s = slice
+    shape = (22, 18, 14)
+    eps_arr = rs.uniform(1.5, 4.0, size=(5, 4, 3)) + 1j * rs.uniform(0.005, 0.05, size=(5, 4, 3))
+    return dict(
+        name="complex_object",
+        shape=shape,
+        grid_spacing=50e-9,
+        courant_number=None,
+        permittivity=1.0,
+        permeability=1.0,
+        init={"E": rs.uniform(-1, 1, size=shape + (3,)), "H": rs.uniform(-1, 1, size=shape + (3,))},
+        components=[
+            ("PML", dict(), (s(0, 3), s(None), s(None)), {}),
+            ("PML", dict(), (s(-3, None), s(None), s(None)), {}),
+            ("PML", dict(), (s(None), s(0, 3), s(None)), {}),
+            ("PML", dict(), (s(None), s(-3, None), s(None)), {}),
+            ("PML", dict(), (s(None), s(None), s(0, 3)), {}),
+            ("Object", dict(permittivity=2.5 + 0.05j, name="lossy"), (s(6, 12), s(5, 11), s(4, 9)), {}),
+            ("Object", dict(permittivity=eps_arr), (s(13, 18), s(4, 8), s(6, 9)), {}),
+            ("Object", dict(permittivity=1.8), (s(4, 8), s(11, 15), s(8, 12)), {}),
+            ("PointSource", dict(period=10, pulse=True), (10, 9, 7), {}),
+            ("LineSource", dict(period=14), (s(5, 17), 13, 10), {}),
+            ("LineDetector", dict(name="det"), (s(3, 19), 9, 10), {}),
+        ],
+        program=[("step", 50)],
+    )
+
+
 def all_scenarios():
     out = [scenario_c1(), scenario_g2(), scenario_manual(), scenario_float_keys(), scenario_1d(),
-           scenario_soft_source()]
+           scenario_soft_source(), scenario_complex_object()]
     for axis in range(3):
         for first in (True, False):
             out.append(_periodic_scenario(f"periodic_{'xyz'[axis]}_{'first' if first else 'last'}", axis, first))
@@ -292,7 +328,11 @@ def input_checksum(sc):
             if isinstance(v, np.ndarray):
                 arrays.append(v)
     for i, a in enumerate(arrays):
-        acc += float(np.sum(np.asarray(a, dtype=np.float64) * (i + 1)))
+        a = np.asarray(a)
+        if np.iscomplexobj(a):  # real and imaginary parts both count
+            acc += float(np.sum(a.imag.astype(np.float64) * (i + 1)))
+            a = a.real
+        acc += float(np.sum(a.astype(np.float64) * (i + 1)))
     return acc
 
 

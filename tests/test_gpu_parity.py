@@ -33,8 +33,10 @@ def _say(msg):
 
 
 def rel_l2(a, b):
-    d = np.linalg.norm((np.asarray(a, np.float64) - np.asarray(b, np.float64)).ravel())
-    n = np.linalg.norm(np.asarray(b, np.float64).ravel())
+    a, b = np.asarray(a), np.asarray(b)
+    dt = np.complex128 if np.iscomplexobj(a) or np.iscomplexobj(b) else np.float64
+    d = np.linalg.norm((a.astype(dt) - b.astype(dt)).ravel())
+    n = np.linalg.norm(b.astype(dt).ravel())
     return d / n if n > 0 else d
 
 
@@ -286,12 +288,6 @@ def test_compensated_float32_error_against_float64_reference(golden, name):
             assert rel_l2(_np(c.trace("E")), g[f"out/{i}/E"]) < 1e-6
 
 
-def rel_c(a, b):
-    """Relative L2 distance of two complex arrays."""
-    a, b = np.asarray(a, np.complex128), np.asarray(b, np.complex128)
-    return np.linalg.norm((a - b).ravel()) / np.linalg.norm(b.ravel())
-
-
 def test_frequency_routines_match_reference(golden):
     """FrequencyRoutines on the device (torch.fft, complex128) after the
     soft-source run, against what the reference's routines computed on its
@@ -303,9 +299,34 @@ def test_frequency_routines_match_reference(golden):
     src = comps[8]
     f, z = fdtd.FrequencyRoutines(grid, src).impedance()
     assert rel_l2(_np(f), g["out/8/impedance_freqs"]) < 1e-15
-    assert rel_c(_np(z), g["out/8/impedance"]) < 1e-12
+    assert rel_l2(_np(z), g["out/8/impedance"]) < 1e-12
     hist = src.trace("source_voltage").reshape(-1)
     f, sp = fdtd.FrequencyRoutines(grid, hist).FFT()
     assert rel_l2(_np(f), g["out/8/fft_freqs"]) < 1e-15
-    assert rel_c(_np(sp), g["out/8/fft"]) < 1e-12
+    assert rel_l2(_np(sp), g["out/8/fft"]) < 1e-12
     assert fdtd.FrequencyRoutines(fdtd.Grid((8, 8, 8), grid_spacing=1e-3), hist).FFT() == ([], [])
+
+
+def test_complex_fields_float32_error_against_reference(golden):
+    """The complex-field path in complex64 (hip.float32) against the
+    reference's complex128 run: error stated, bounded at 1e-5 relative."""
+    g = golden("complex_object")
+    fdtd.set_backend("hip.float32")
+    sc = scenarios.get_scenario("complex_object")
+    grid, comps = _build(sc)
+    assert grid.complex and grid.E.dtype == torch.complex64
+    scenarios.run_program(grid, sc)
+    eE, eH = rel_l2(_np(grid.E), g["E"]), rel_l2(_np(grid.H), g["H"])
+    _say(f"[gpu] complex64 vs reference complex128: E rel {eE:.2e}, H rel {eH:.2e}")
+    assert eE <= 1e-5 and eH <= 1e-5
+
+
+def test_complex_fields_pml_registered_late_fails_like_the_reference():
+    """A PML registered after the fields became complex keeps real CPML
+    state; the reference's first step then fails casting complex into it."""
+    grid = fdtd.Grid((12, 10, 8), grid_spacing=50e-9)
+    grid[0:2, :, :] = fdtd.PML()
+    grid[4:7, 3:6, 2:5] = fdtd.Object(permittivity=2.0 + 0.1j)
+    grid[-2:, :, :] = fdtd.PML()
+    with pytest.raises(TypeError):
+        grid.step()

@@ -203,3 +203,23 @@ def test_stepping_refuses_host_arrays():
     grid = _grid()
     with pytest.raises(RuntimeError):
         grid.step()
+
+
+def test_complex_permittivity_promotes_fields_and_cpml_state():
+    """objects.py:51-52 / grid.py:353-356: an Object with a complex
+    permittivity makes E, H and the PML state registered before it complex;
+    periodic boundaries and absorbing objects are refused as documented."""
+    grid = fdtd.Grid((12, 10, 8), grid_spacing=50e-9)
+    grid[0:2, :, :] = fdtd.PML()
+    grid[4:7, 3:6, 2:5] = fdtd.Object(permittivity=2.0 + 0.1j)
+    assert grid.complex and grid.E.dtype == torch.complex128
+    assert grid.boundaries[0]._psiE.dtype == torch.complex128
+    inv = grid.objects[0].inverse_permittivity
+    assert inv.dtype == torch.complex128
+    assert inv[0, 0, 0, 0].item() == np.complex128(1) / np.complex128(2.0 + 0.1j)
+    with pytest.raises(NotImplementedError):
+        grid[8:10, 3:6, 2:5] = fdtd.AbsorbingObject(permittivity=1.5, conductivity=1e3)
+    g2 = fdtd.Grid((12, 10, 8), grid_spacing=50e-9)
+    g2[:, :, 0] = fdtd.PeriodicBoundary()
+    with pytest.raises(AttributeError):
+        g2[4:7, 3:6, 2:5] = fdtd.Object(permittivity=2.0 + 0.1j)

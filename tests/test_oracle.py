@@ -173,3 +173,35 @@ def test_frequency_routines_match_reference_fixture(golden):
     assert np.array_equal(f, g["out/8/impedance_freqs"]) and np.array_equal(z, g["out/8/impedance"])
     f, sp = orc.fr_fft(g["out/8/source_voltage"], dt)
     assert np.array_equal(f, g["out/8/fft_freqs"]) and np.array_equal(sp, g["out/8/fft"])
+
+
+def test_complex_path_pins_numpys_complex_product(golden, monkeypatch):
+    """The complex fixture (written by the reference) fixes numpy's complex
+    product, re = fma(ar, br, -(ai*bi)), im = fma(ar, bi, ai*br): the oracle
+    (numpy) reproduces it bit for bit, and the textbook product without
+    fused multiply-adds -- a plausible restatement -- does not."""
+    g = golden("complex_object")
+    sc, sim, comps = _run("complex_object")
+    assert np.iscomplexobj(sim.E) and np.iscomplexobj(comps[0].psi_E[1])
+    scenarios.run_program(sim, sc)
+    assert np.array_equal(sim.E, g["E"]) and np.array_equal(sim.H, g["H"])
+
+    def textbook(self, curl):
+        s = self.sim
+        loc = (self.x, self.y, self.z)
+        k = s.courant_number * self.inverse_permittivity
+        c = curl[loc]
+        s.E[loc] = s.E[loc] + ((k.real * c.real - k.imag * c.imag) + 1j * (k.real * c.imag + k.imag * c.real))
+
+    monkeypatch.setattr(orc._Object, "update_E", textbook)
+    sc, sim, comps = _run("complex_object")
+    scenarios.run_program(sim, sc)
+    assert not np.array_equal(sim.E, g["E"])
+    assert np.abs(sim.E - g["E"]).max() < 1e-12 * np.abs(g["E"]).max()
+
+
+def test_complex_permittivity_with_periodic_boundary_fails_like_the_reference():
+    sim = orc.Sim((8, 8, 8), grid_spacing=50e-9)
+    sim.add("PeriodicBoundary", (slice(None), slice(None), 0))
+    with pytest.raises(AttributeError):
+        sim.add("Object", (slice(2, 5), slice(2, 5), slice(2, 5)), permittivity=2.0 + 0.1j)
