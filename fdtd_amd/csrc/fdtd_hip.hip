@@ -886,7 +886,13 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     const int bzs = lanes_shift(p->d.nz);
     const int bz = 1 << bzs, by = std::max(1, p->block_threads >> bzs);
     const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by);
-    int64_t cx = tiles * p->d.nx / 32768;
+    // measured on MI355X: plain storage is fastest at 2 planes per workgroup
+    // from 256^3 up to 512x512x384 (14 % faster there than at 12) and at 1 on
+    // grids of ~1M cells; the compensated kernels prefer ~48K workgroups (2 at
+    // 256^3, 8 at 512x512x384)
+    const int64_t work = tiles * p->d.nx;
+    int64_t cx = std::min<int64_t>(2, work / 32768);
+    if (p->dtype == FDTD_F32C && work >= 65536) cx = std::max<int64_t>(2, std::min<int64_t>(8, (work + 24576) / 49152));
     if (const char* env = getenv("FDTD_HIP_CX")) cx = atoi(env);
     if (const char* env = getenv("FDTD_HIP_NT")) p->nontemporal = atoi(env) != 0;
     // cyclic dealing pays once the fields outgrow the caches (16.7M cells = 256^3)
