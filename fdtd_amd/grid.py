@@ -561,5 +561,38 @@ class Grid:
     def generate_video(self, delete_frames=False):
         raise NotImplementedError("video generation (matplotlib + ffmpeg) is not part of the HIP backend")
 
+    def grid_energy(self, x=None, y=None, z=None):
+        """The optical energy plane that the reference's visualize() draws
+        (visualization.py:86-146): sum over the components of E**2 + H**2,
+        at one x, y or z index, computed on the device for that plane only.
+        The y plane is returned transposed, (z, x), as it is drawn there; the
+        argument checks raise the same errors."""
+        checks = (("x", x, (y, z), "an `x`", "a `y` or a `z`"), ("y", y, (z, x), "a `y`", "a `z` or a `x`"),
+                  ("z", z, (x, y), "a `z`", "a `x` or a `y`"))
+        for name, v, others, this, rest in checks:
+            if v is None:
+                continue
+            if not isinstance(v, int):
+                raise ValueError(f"the `{name}`-location supplied should be a single integer")
+            if any(o is not None for o in others):
+                raise ValueError(f"if {this}-location is supplied, one should not supply {rest}-location!")
+            break
+        else:
+            raise ValueError("at least one projection plane (x, y or z) should be supplied to visualize the grid!")
+        E, H = self.E, self.H
+        if x is not None:
+            assert self.Ny > 1 and self.Nz > 1
+            e, h = E[x], H[x]
+        elif y is not None:
+            assert self.Nx > 1 and self.Nz > 1
+            e, h = E[:, y], H[:, y]
+        else:
+            assert self.Nx > 1 and self.Ny > 1
+            e, h = E[:, :, z], H[:, :, z]
+        sq = e ** 2 + h ** 2
+        energy = (sq[..., 0] + sq[..., 1]) + sq[..., 2]  # numpy's order for a 3-term sum
+        return energy.T if y is not None else energy
+
     def visualize(self, *args, **kwargs):
-        raise NotImplementedError("visualization is not part of the HIP backend; use bd.numpy(grid.E)")
+        raise NotImplementedError("plotting is not part of the HIP backend: draw grid.grid_energy(x=..|y=..|z=..) "
+                                  "(the energy plane visualize() shows, computed on the device) with matplotlib")

@@ -223,3 +223,27 @@ def test_complex_permittivity_promotes_fields_and_cpml_state():
     g2[:, :, 0] = fdtd.PeriodicBoundary()
     with pytest.raises(AttributeError):
         g2[4:7, 3:6, 2:5] = fdtd.Object(permittivity=2.0 + 0.1j)
+
+
+@pytest.mark.parametrize("name", ["g2", "complex_object"])
+def test_grid_energy_is_the_visualize_reduction(golden, name):
+    """grid_energy(x|y|z) equals the reference's visualize() energy,
+    sum(E**2 + H**2, -1) sliced (transposed for y), on the fixture fields:
+    bit for bit for real fields; for complex ones within 1e-15 relative
+    (numpy squares complex numbers with a fused multiply-add, torch does not)."""
+    g = golden(name)
+    sc = scenarios.get_scenario(name)
+    grid, _ = scenarios.build_fdtd(fdtd, sc, to_array=torch.from_numpy)
+    grid.E = torch.from_numpy(g["E"])
+    grid.H = torch.from_numpy(g["H"])
+    ref = np.sum(g["E"] ** 2 + g["H"] ** 2, -1)
+    for got, want in ((grid.grid_energy(x=3), ref[3, :, :]), (grid.grid_energy(y=4), ref[:, 4, :].T),
+                      (grid.grid_energy(z=5), ref[:, :, 5])):
+        if np.iscomplexobj(want):
+            np.testing.assert_allclose(_np(got), want, rtol=1e-15, atol=1e-15 * np.abs(want).max())
+        else:
+            assert np.array_equal(_np(got), want)
+    with pytest.raises(ValueError):
+        grid.grid_energy(x=1, z=2)
+    with pytest.raises(ValueError):
+        grid.grid_energy()
