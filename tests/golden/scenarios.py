@@ -416,6 +416,34 @@ def build_oracle(oracle_mod, sc):
     return sim, comps
 
 
+def random_complex_scenario(seed, steps=15):
+    """random_scenario(seed) moved onto the complex-field path: periodic
+    boundaries dropped (the reference cannot promote them), the PML faces
+    registered first (a PML registered after the promotion fails there),
+    every object a plain Object, the first of them with a complex
+    permittivity (scalar or per cell, imaginary part <= 0.05: the complex
+    leap-frog grows otherwise), one added when the scenario has none."""
+    sc = random_scenario(seed, steps)
+    rs = np.random.RandomState(5000 + seed)
+    comps = [c for c in sc["components"] if c[0] != "PeriodicBoundary"]
+    pmls = [c for c in comps if c[0] == "PML"]
+    objs = [("Object", dict(permittivity=c[1]["permittivity"]), c[2], {}) for c in comps if c[0].endswith("Object")]
+    rest = [c for c in comps if c[0] != "PML" and not c[0].endswith("Object")]
+    if not objs:
+        shape = sc["shape"]
+        key = tuple(slice(n // 3, max(n // 3 + 1, 2 * n // 3)) for n in shape)
+        objs.append(("Object", dict(permittivity=2.0), key, {}))
+    perm = objs[0][1]["permittivity"]
+    if np.ndim(perm) == 0:
+        perm = complex(perm, float(rs.uniform(0.005, 0.05)))
+    else:
+        perm = perm + 1j * rs.uniform(0.005, 0.05, size=np.shape(perm))
+    objs[0] = ("Object", dict(permittivity=perm), objs[0][2], {})
+    sc["components"] = pmls + objs + rest
+    sc["name"] = f"random_complex_{seed}"
+    return sc
+
+
 def random_scenario(seed, steps=15):
     """A random but valid configuration: random grid (sometimes 2D), random
     subset and registration order of PML faces, an optional periodic boundary

@@ -330,3 +330,28 @@ def test_complex_fields_pml_registered_late_fails_like_the_reference():
     grid[-2:, :, :] = fdtd.PML()
     with pytest.raises(TypeError):
         grid.step()
+
+
+@pytest.mark.parametrize("seed", range(6))
+def test_complex_random_scenarios_match_oracle(seed):
+    """Random configurations on the complex-field path (complex128) against
+    the oracle's numpy complex arithmetic, bit for bit; the energy plane of
+    visualize() computed on the device against numpy's."""
+    sc = scenarios.random_complex_scenario(seed)
+    _say(f"[gpu] random complex scenario {seed}: shape {sc['shape']}, {[c[0] for c in sc['components']]}")
+    sim, ocomps = scenarios.build_oracle(orc, sc)
+    scenarios.run_program(sim, sc)
+    grid, comps = _build(sc)
+    assert grid.complex
+    scenarios.run_program(grid, sc)
+    torch.cuda.synchronize()
+    _assert_same(_np(grid.E), sim.E, "E")
+    _assert_same(_np(grid.H), sim.H, "H")
+    dets = [c for c in comps if isinstance(c, (fdtd.LineDetector, fdtd.BlockDetector))]
+    for d, o in zip(dets, sim.detectors):
+        _assert_same(_np(d.trace("E")).reshape(np.array(o.E).shape), np.array(o.E), "detector E")
+        _assert_same(_np(d.trace("H")).reshape(np.array(o.H).shape), np.array(o.H), "detector H")
+    z = sc["shape"][2] // 2
+    if sc["shape"][0] > 1 and sc["shape"][1] > 1:
+        ref = np.sum(sim.E ** 2 + sim.H ** 2, -1)[:, :, z]
+        np.testing.assert_allclose(_np(grid.grid_energy(z=z)), ref, rtol=1e-15, atol=1e-15 * np.abs(ref).max())
