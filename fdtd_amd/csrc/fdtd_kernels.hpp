@@ -282,12 +282,21 @@ struct CompField {
 };
 
 // --------------------------------------------------------------- materials
+// Materials are read in two parts: key(i) -- what can be loaded a plane
+// ahead (the palette index) -- and get3(key, i). The marching kernels load
+// the next plane's key with the current plane, so a palette lookup does not
+// wait for its index.
+struct NoKey {};
+
 template <typename S, typename R>
 struct MatArray {
+  using Key = NoKey;
   const S* p;  // (3, N)
   int64_t cs;
   template <bool NT>
-  __device__ __forceinline__ void get3(int64_t i, R m[3]) const {
+  __device__ __forceinline__ Key key(int64_t) const { return {}; }
+  template <bool NT>
+  __device__ __forceinline__ void get3(Key, int64_t i, R m[3]) const {
 #pragma unroll
     for (int c = 0; c < 3; ++c) m[c] = (R)ld_stream<NT>(p + c * cs + i);
   }
@@ -295,11 +304,13 @@ struct MatArray {
 
 template <typename R>
 struct MatPalette {
+  using Key = int;
   const uint8_t* id;  // (N)
   const double* tab;  // (256, 3)
   template <bool NT>
-  __device__ __forceinline__ void get3(int64_t i, R m[3]) const {
-    const int k = (int)ld_stream<NT>(id + i);
+  __device__ __forceinline__ Key key(int64_t i) const { return (int)ld_stream<NT>(id + i); }
+  template <bool NT>
+  __device__ __forceinline__ void get3(Key k, int64_t, R m[3]) const {
 #pragma unroll
     for (int c = 0; c < 3; ++c) m[c] = (R)gp(tab)[3 * k + c];
   }
@@ -520,12 +531,15 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
     hz_prev = a.G.template get_c<2>(cell - sx);
     hy_prev = a.G.template get_c<1>(cell - sx);
   }
+  typename MS::Key key = a.M.template key<NT>(cell);
   for (int i = i0; i < i1; ++i, cell += sx) {
     T hv[3], m[3], e[3];
+    const typename MS::Key kc_ = key;
+    if (i + 1 < i1) key = a.M.template key<NT>(cell + sx);  // the next plane's, loaded with this one
     a.G.template get3_s<false>(cell, hv);
     const T hy_km = a.G.template get_c<1>(cell - zm), hx_km = a.G.template get_c<0>(cell - zm);
     const T hz_jm = a.G.template get_c<2>(cell - ym), hx_jm = a.G.template get_c<0>(cell - ym);
-    a.M.template get3<NT>(cell, m);
+    a.M.template get3<NT>(kc_, cell, m);
     a.F.template get3_s<NT>(cell, e);
     const T hx = hv[0], hy = hv[1], hz = hv[2];
     // backward differences; an absent one contributes nothing (grid.py:67-74)
@@ -616,12 +630,15 @@ __device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__
     ez_next = a.G.template get_c<2>(cell + sx);
     ey_next = a.G.template get_c<1>(cell + sx);
   }
+  typename MS::Key key = a.M.template key<NT>(cell);
   for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
     T ev[3], m[3], h[3];
+    const typename MS::Key kc_ = key;
+    if (i > i0) key = a.M.template key<NT>(cell - sx);  // the next plane's (marching -x)
     a.G.template get3_s<false>(cell, ev);
     const T ey_kp = a.G.template get_c<1>(cell + zp), ex_kp = a.G.template get_c<0>(cell + zp);
     const T ez_jp = a.G.template get_c<2>(cell + yp), ex_jp = a.G.template get_c<0>(cell + yp);
-    a.M.template get3<NT>(cell, m);
+    a.M.template get3<NT>(kc_, cell, m);
     a.F.template get3_s<NT>(cell, h);
     const T ex = ev[0], ey = ev[1], ez = ev[2];
     // forward differences; an absent one contributes nothing (grid.py:42-49)
@@ -744,7 +761,7 @@ __global__ void __launch_bounds__(256) k_face(Dims d, typename FS::R sc, FS F, F
     d1 = G.get(o1 + st) - G.get(o1);
   }
   T m[3], kc[3], v[3];
-  M.template get3<false>(cell, m);
+  M.template get3<false>(M.template key<false>(cell), cell, m);
 #pragma unroll
   for (int c = 0; c < 3; ++c) {
     kc[c] = sc * m[c];
