@@ -70,8 +70,12 @@ class HipBackend(Backend):
 
     int = torch.int64
     float = torch.float64
-    complex = torch.complex128
     compensated = False
+
+    @property
+    def complex(self):
+        """complex type of the fields after promotion (backend.py:216-220)"""
+        return torch.complex64 if self.float == torch.float32 else torch.complex128
 
     @property
     def material_float(self):

@@ -32,6 +32,11 @@
 //                        after a periodic boundary (order matters there).
 //   k_periodic           plane copy of a periodic boundary.
 //   k_tail               one workgroup: sources (in order) then detectors.
+//   k_batch_E / k_batch_H  the main kernels' tile bodies over the tiles of
+//                        many grids (one launch per half step); k_set_time
+//                        sets their device time index.
+//   k_adj_H / k_adj_E    transposes of the H and E half steps (reverse mode),
+//                        objective and design gradient fused.
 //   k_copy               plain streaming copy (bandwidth reference).
 #pragma once
 
