@@ -601,11 +601,19 @@ __device__ __forceinline__ int64_t launch_tile(const MainArgs<FS, MS>& a, unsign
   return a.cyclic ? xcd_chunk_tile(blockIdx.x, per, nch) : (int64_t)xcd_tile(blockIdx.x, gridDim.x);
 }
 
-// At least 6 waves per SIMD: the object variants otherwise take ~118 VGPRs
-// (4 waves) for the anisotropic 3x3 path; under this bound they fit in ~76
-// without spilling, and the other variants are below it anyway.
+// Occupancy floor of the E kernels: the object variants otherwise take ~118
+// VGPRs (4 waves per SIMD) for the anisotropic 3x3 path. Under a floor of 6
+// the double-precision ones fit in ~76 VGPRs without spilling, the float32
+// ones under a floor of 8 in ~46 (a floor of 8 makes the double ones spill);
+// complex128 values need twice the registers and keep a floor of 4.
+template <class FS>
+struct EWaves {
+  static constexpr int value = sizeof(typename FS::R) == 4 ? 8 : (sizeof(typename FS::R) == 8 ? 6 : 4);
+};
+
 template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_main_E(MainArgs<FS, MS> a) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(EWaves<FS>::value)))
+k_main_E(MainArgs<FS, MS> a) {
   unsigned n;
   const int64_t t = launch_tile(a, n);
   if (t >= 0) main_E_tile<FS, MS, O, OBJS, NT, REC>(a, (unsigned)t, a.q);
@@ -701,7 +709,8 @@ struct BatchArgs {
 };
 
 template <class FS, class MS, typename O, bool OBJS, bool NT>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(6))) k_batch_E(BatchArgs<FS, MS> b) {
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(EWaves<FS>::value)))
+k_batch_E(BatchArgs<FS, MS> b) {
   const int64_t q = *b.q;
   for (int64_t it = xcd_tile(blockIdx.x, gridDim.x); it < b.total; it += gridDim.x) {
     // the item and its grid are the same for the whole workgroup: say so,
