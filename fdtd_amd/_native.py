@@ -58,6 +58,7 @@ _SIGNATURES = {
     "fdtd_adjoint_destroy": ([_vp], _c_int),
     "fdtd_run_recorded": ([_vp, _c_i64, _c_i64, _vp, _vp, _vp], _c_int),
     "fdtd_adjoint_run": ([_vp, _c_i64, _vp, _vp, _vp], _c_int),
+    "fdtd_adjoint_set_launch": ([_vp, _c_int], _c_int),
 }
 
 
@@ -250,6 +251,10 @@ class Adjoint:
         check(self._L.fdtd_run_recorded(self._h, q0, nsteps,
                                         curl_records.data_ptr() if curl_records is not None else None,
                                         plane_records.data_ptr(), stream))
+
+    def set_launch(self, cx):
+        """x planes per workgroup of the transposed kernels (0: the plan's)."""
+        check(self._L.fdtd_adjoint_set_launch(self._h, int(cx)))
 
     def run_adjoint(self, nsteps, curl_records, plane_records, stream):
         check(self._L.fdtd_adjoint_run(self._h, nsteps, curl_records.data_ptr(), plane_records.data_ptr(), stream))

@@ -20,7 +20,10 @@ steps (one forward pass plus one reverse pass). ``checkpoint="sqrt"`` keeps
 a copy of the whole state (E, H, every CPML psi) every K = ceil(sqrt(T))
 steps instead; the reverse pass restores each checkpoint, steps its segment
 forward again recording K steps, and runs the transposed steps over it (two
-forward passes plus one reverse pass, O(sqrt(T)) states of memory).
+forward passes plus one reverse pass, O(sqrt(T)) states of memory). The last
+RECORD_BUFFERS segments are recorded by the first forward pass into the
+record buffers the reverse pass uses, so they are neither checkpointed nor
+recomputed.
 ``"auto"`` picks "record" when the records fit in a quarter of the free
 device memory.
 """
@@ -33,6 +36,13 @@ import torch
 from . import _native
 from .boundaries import PML
 from .objects import AbsorbingObject, AnisotropicObject, Object
+
+# checkpointed mode: record buffers of K steps (the last RECORD_BUFFERS
+# segments are recorded by the forward pass, the others recomputed into them
+# in turn), and x planes per workgroup of the transposed kernels running
+# beside the recomputation (measured on MI355X at 128x128x64: 2 beats 1 and 4)
+RECORD_BUFFERS = 4
+SQRT_MODE_CX = 2
 
 
 def _box_of(grid, key):
@@ -210,24 +220,29 @@ class IntensityGradient:
         return self._run(permittivity, gradient=True)
 
     def _reverse_checkpointed(self, adj, segments, checkpoints, curl, plane):
-        """Reverse sweep of the checkpointed mode on two streams: segment s-1
-        is recomputed from its checkpoint (forward kernels, recording) on a
-        second stream while the transposed steps of segment s run on the
-        current one. Both are latency-bound kernels on disjoint data (the
+        """Reverse sweep of the checkpointed mode on two streams: the segments
+        are recomputed from their checkpoints (forward kernels, recording) on
+        a second stream while the transposed steps of later segments run on
+        the current one. Both are latency-bound kernels on disjoint data (the
         recomputation writes the grid state and one record buffer, the
-        transposed steps read the other buffer and write the adjoint state),
-        so they overlap; events order each buffer's reuse."""
+        transposed steps read another buffer and write the adjoint state), so
+        they overlap. Segment s uses record buffer s % len(curl); the last
+        len(curl) segments were recorded by the forward pass on the current
+        stream, checkpoints[s] restores each earlier one. Events order each
+        buffer's reuse: a recomputation is issued only after the transposed
+        steps that last read its buffer, so that its wait sees them."""
         cur = torch.cuda.current_stream()
         side = getattr(self, "_side", None)
         if side is None or side.device != cur.device:
             side = self._side = torch.cuda.Stream(device=cur.device)
+        nbuf = len(curl)
         final = [t.clone() for t in self._state()]
         side.wait_stream(cur)  # the forward pass and the final-state copy are done
         recorded = [None] * len(segments)
-        consumed = [None, None]  # per buffer: the transposed steps that last read it
+        consumed = [None] * nbuf  # per buffer: the transposed steps that last read it
 
         def recompute(s):
-            b = s % 2
+            b = s % nbuf
             with torch.cuda.stream(side):
                 if consumed[b] is not None:
                     side.wait_event(consumed[b])
@@ -239,12 +254,15 @@ class IntensityGradient:
                 recorded[s].record(side)
 
         last = len(segments) - 1
-        recompute(last)
+        pending = len(checkpoints) - 1  # the next segment to recompute, downwards
         for s in range(last, -1, -1):
-            if s > 0:
-                recompute(s - 1)
-            b = s % 2
-            cur.wait_event(recorded[s])
+            # buffer (s' % nbuf) was last read by segment s' + nbuf >= s + 1
+            while pending >= 0 and pending >= s + 1 - nbuf:
+                recompute(pending)
+                pending -= 1
+            b = s % nbuf
+            if recorded[s] is not None:
+                cur.wait_event(recorded[s])
             adj.run_adjoint(segments[s][1], curl[b], plane[b], cur.cuda_stream)
             consumed[b] = torch.cuda.Event()
             consumed[b].record(cur)
@@ -271,23 +289,31 @@ class IntensityGradient:
         psi = torch.zeros(_native.Adjoint.psi_size(plan), dtype=dt, device=dev)
         grad = torch.zeros((3, self.nd), dtype=torch.float64, device=dev)
         adj = _native.Adjoint(plan, gcoef, dcoef, lamE, lamH, psi, self.design_box, self.box, grad)
-        # two record buffers: in the checkpointed mode the recomputation of
-        # segment s-1 fills one while the transposed steps of segment s read
-        # the other
-        nbuf = 2 if gradient and mode == "sqrt" else 1
+        # in the checkpointed mode RECORD_BUFFERS buffers of K steps: the
+        # recomputation of earlier segments fills some while the transposed
+        # steps of a later one read another; fewer and longer workgroups of
+        # the transposed kernels leave room for the recomputation beside them
+        sqrt_grad = gradient and mode == "sqrt"
+        nbuf = RECORD_BUFFERS if sqrt_grad else 1
+        adj.set_launch(SQRT_MODE_CX if sqrt_grad else 0)
         curl = [torch.empty((K, 3, self.nd), dtype=dt, device=dev) for _ in range(nbuf)]
         plane = [torch.empty((K, 3, self.np), dtype=dt, device=dev) for _ in range(nbuf)]
         J = torch.zeros((), dtype=torch.float64, device=dev)
 
         self._reset()
         segments = [(q, min(K, T - q)) for q in range(0, T, K)]
+        last = len(segments) - 1
         checkpoints = []
-        for q, m in segments:
-            if gradient and mode == "sqrt":
+        for s, (q, m) in enumerate(segments):
+            b = s % nbuf
+            # checkpointed gradient: the last nbuf segments are recorded now,
+            # into the buffers the reverse sweep reads them from, and are
+            # not recomputed; the others keep a checkpoint instead
+            keep = mode == "record" or (sqrt_grad and s > last - nbuf)
+            if sqrt_grad and not keep:
                 checkpoints.append([t.clone() for t in self._state()])
-            # the curl is needed now only when nothing is recomputed later
-            adj.run_recorded(q, m, curl[0] if mode == "record" else None, plane[0], stream)
-            J += plane[0][:m].to(torch.float64).square().sum()
+            adj.run_recorded(q, m, curl[b] if keep else None, plane[b], stream)
+            J += plane[b][:m].to(torch.float64).square().sum()
         g.time_steps_passed = T
         self.last_info = {"mode": mode, "segment": K, "checkpoints": len(checkpoints)}
         if not gradient:

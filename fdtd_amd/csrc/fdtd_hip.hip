@@ -152,8 +152,10 @@ struct FdtdAdjoint {
   void* psi = nullptr;
   Box db{}, pb{};
   double* grad = nullptr;
-  AdjFaceDev* d_afaces = nullptr;  // device table
+  // AdjSide table in device memory: [half step (H, E)][parity][axis][side]
+  void* d_sides = nullptr;
   int parE = 0, parH = 0;
+  int cx = 0;  // x planes per workgroup of the transposed kernels (0: the plan's)
 };
 
 namespace {
@@ -450,20 +452,50 @@ AdjArgs<T> adj_args(const FdtdAdjoint* a) {
   r.inv_mu = reinterpret_cast<const T*>(p->inv_mu);
   r.gcoef = reinterpret_cast<const T*>(a->gcoef);
   r.dcoef = reinterpret_cast<const T*>(a->dcoef);
-  r.faces = p->d_faces;
-  r.afaces = a->d_afaces;
-  r.nfaces = (int32_t)p->faces.size();
-  for (int ax = 0; ax < 3; ++ax) r.face_of[ax][0] = r.face_of[ax][1] = -1;
-  for (size_t f = 0; f < p->faces.size(); ++f) r.face_of[p->faces[f].axis][p->faces[f].high ? 1 : 0] = (int32_t)f;
   r.db = a->db;
   r.pb = a->pb;
   r.grad = a->grad;
   int bzs = 0;
   while ((1 << bzs) < p->d.nz && bzs < 6) ++bzs;
   r.bz_shift = bzs;
-  r.cx = p->cx;
+  int cx = a->cx > 0 ? a->cx : p->cx;
+  if (const char* env = getenv("FDTD_HIP_ADJ_CX")) cx = atoi(env);
+  r.cx = std::max(1, std::min(cx, p->d.nx));
   r.by = std::max(1, p->block_threads >> bzs);
   return r;
+}
+
+// The CPML faces of the transposed half steps (H: psiH, bH, cH; E: psiE,
+// bE, cE), for both parities: buffer `parity` is read, the other written.
+// Face f's adjoint buffers in psi_adj: psiE[0], psiE[1], psiH[0], psiH[1],
+// (2, slab) values each.
+template <typename T>
+std::vector<AdjSide<T>> adj_side_table(const FdtdPlan* p, void* psi_adj) {
+  std::vector<AdjSide<T>> tab(2 * 2 * 6);
+  T* base = reinterpret_cast<T*>(psi_adj);
+  int64_t off = 0;
+  for (size_t f = 0; f < p->faces.size(); ++f) {
+    const FaceDev& fd = p->h_faces[f];
+    for (int half = 0; half < 2; ++half) {  // 0: H, 1: E
+      T* buf0 = base + off + (half ? 0 : 4 * fd.slab);
+      for (int par = 0; par < 2; ++par) {
+        AdjSide<T>& e = tab[(half * 2 + par) * 6 + fd.axis * 2 + (fd.high ? 1 : 0)];
+        e.pin = buf0 + 2 * par * fd.slab;
+        e.pout = buf0 + 2 * (1 - par) * fd.slab;
+        e.b = half ? fd.bE : fd.bH;
+        e.c = half ? fd.cE : fd.cH;
+        e.slab = fd.slab;
+        e.t = fd.t;
+      }
+    }
+    off += 8 * fd.slab;
+  }
+  return tab;
+}
+
+template <typename T>
+const AdjSide<T>* adj_sides(const FdtdAdjoint* a, bool is_H, int parity) {
+  return reinterpret_cast<const AdjSide<T>*>(a->d_sides) + ((is_H ? 0 : 2) + parity) * 6;
 }
 
 template <typename T>
@@ -476,11 +508,11 @@ int adjoint_steps(FdtdAdjoint* a, int64_t nsteps, const T* curl, const T* plane,
   const dim3 grid(tiles * (unsigned)((p->d.nx + args.cx - 1) / args.cx)), block(bz * by);
   for (int64_t s = nsteps - 1; s >= 0; --s) {
     args.plane = plane + s * 3 * np;
-    args.parity = a->parH;
+    args.side = adj_sides<T>(a, true, a->parH);
     hipLaunchKernelGGL(k_adj_H<T>, grid, block, 0, st, args);
     a->parH ^= 1;
     args.curl = curl + s * 3 * nd;
-    args.parity = a->parE;
+    args.side = adj_sides<T>(a, false, a->parE);
     hipLaunchKernelGGL(k_adj_E<T>, grid, block, 0, st, args);
     a->parE ^= 1;
   }
@@ -1088,17 +1120,26 @@ int fdtd_adjoint_create(FdtdPlan* p, const void* gcoef, const void* dcoef, void*
   if (!box_ok(p, objective_box)) return fail(-1, "objective box outside the grid or empty");
   if (!p->faces.empty() && !psi_adj) return fail(-1, "null adjoint CPML buffer");
   if (int rc = select_device(p)) return rc;
-  const size_t esz = p->dtype == FDTD_F64 ? sizeof(double) : sizeof(float);
-  std::vector<AdjFaceDev> tab(p->faces.size());
-  char* base = reinterpret_cast<char*>(psi_adj);
-  int64_t off = 0;
-  for (size_t i = 0; i < p->faces.size(); ++i) {
-    const int64_t slab = p->h_faces[i].slab;
-    for (int b = 0; b < 2; ++b) tab[i].psiE[b] = base + (off + 2 * b * slab) * esz;
-    for (int b = 0; b < 2; ++b) tab[i].psiH[b] = base + (off + 4 * slab + 2 * b * slab) * esz;
-    off += 8 * slab;
-  }
   FdtdAdjoint* a = new FdtdAdjoint();
+  {
+    std::vector<unsigned char> tab;
+    if (p->dtype == FDTD_F64) {
+      const auto t = adj_side_table<double>(p, psi_adj);
+      tab.assign(reinterpret_cast<const unsigned char*>(t.data()),
+                 reinterpret_cast<const unsigned char*>(t.data() + t.size()));
+    } else {
+      const auto t = adj_side_table<float>(p, psi_adj);
+      tab.assign(reinterpret_cast<const unsigned char*>(t.data()),
+                 reinterpret_cast<const unsigned char*>(t.data() + t.size()));
+    }
+    hipError_t e = hipMalloc(&a->d_sides, tab.size());
+    if (e == hipSuccess) e = hipMemcpy(a->d_sides, tab.data(), tab.size(), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+      if (a->d_sides) (void)hipFree(a->d_sides);
+      delete a;
+      return fail(-3, std::string("adjoint face table: ") + hipGetErrorString(e));
+    }
+  }
   a->plan = p;
   a->gcoef = gcoef;
   a->dcoef = dcoef;
@@ -1109,24 +1150,15 @@ int fdtd_adjoint_create(FdtdPlan* p, const void* gcoef, const void* dcoef, void*
   a->pb = Box{objective_box[0], objective_box[1], objective_box[2], objective_box[3], objective_box[4],
               objective_box[5]};
   a->grad = grad;
-  if (!tab.empty()) {
-    hipError_t e = hipMalloc(&a->d_afaces, tab.size() * sizeof(AdjFaceDev));
-    if (e == hipSuccess) e = hipMemcpy(a->d_afaces, tab.data(), tab.size() * sizeof(AdjFaceDev), hipMemcpyHostToDevice);
-    if (e != hipSuccess) {
-      if (a->d_afaces) (void)hipFree(a->d_afaces);
-      delete a;
-      return fail(-3, std::string("adjoint face table: ") + hipGetErrorString(e));
-    }
-  }
   *out = a;
   return 0;
 }
 
 int fdtd_adjoint_destroy(FdtdAdjoint* a) {
   if (!a) return 0;
-  if (a->d_afaces) {
+  if (a->d_sides) {
     (void)hipDeviceSynchronize();
-    (void)hipFree(a->d_afaces);
+    (void)hipFree(a->d_sides);
   }
   delete a;
   return 0;
@@ -1147,6 +1179,13 @@ int fdtd_run_recorded(FdtdAdjoint* a, int64_t q0, int64_t nsteps, void* curl_rec
                                   reinterpret_cast<double*>(plane_records), st);
   return recorded_steps<float>(a, q0, nsteps, reinterpret_cast<float*>(curl_records),
                                reinterpret_cast<float*>(plane_records), st);
+}
+
+int fdtd_adjoint_set_launch(FdtdAdjoint* a, int cx) {
+  if (!a) return fail(-1, "null adjoint");
+  if (cx < 0) return fail(-1, "cx must be 0 (the plan's) or positive");
+  a->cx = cx;
+  return 0;
 }
 
 int fdtd_adjoint_run(FdtdAdjoint* a, int64_t nsteps, const void* curl_records, const void* plane_records,

@@ -941,12 +941,21 @@ __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uin
 // the E kernel), the transpose of the E half step forward ones (the shape of
 // the H kernel).
 
-// Adjoint CPML state of one face: psi of the E and of the H side, two
-// buffers each, read and written alternately (input `parity`, output
-// 1 - parity), (2, slab) values per buffer like the forward psi.
-struct AdjFaceDev {
-  void* psiE[2];
-  void* psiH[2];
+// One CPML face as a transposed kernel sees it at one launch: the adjoint
+// psi of its half step that the launch reads and the one it writes (two
+// buffers used alternately, (2, slab) values each like the forward psi), the
+// half step's b and c profiles (t values) and the slab size; t = 0 when the
+// side has no face. A device table holds one per half step, parity, axis and
+// side, so a kernel reads each face's entry with one batch of scalar loads
+// and no pointer chasing.
+template <typename T>
+struct AdjSide {
+  const T* pin;
+  T* pout;
+  const double* b;
+  const double* c;
+  int64_t slab;
+  int32_t t, pad;
 };
 
 template <typename T>
@@ -959,11 +968,7 @@ struct AdjArgs {
   const T* inv_mu;
   const T* gcoef;        // (3, N)
   const T* dcoef;        // (3, N) or null (all ones)
-  const FaceDev* faces;  // CPML faces (b, c profiles)
-  const AdjFaceDev* afaces;
-  int32_t nfaces;
-  int32_t parity;
-  int32_t face_of[3][2];  // face index on the low / high side of each axis, or -1
+  const AdjSide<T>* side;  // [axis][low/high side] of this launch's half step and parity
   Box db;                // design box: grad += S lamE curl
   const T* curl;         // (3, cells of db) recorded for this step
   double* grad;          // (3, cells of db)
@@ -993,7 +998,7 @@ __device__ __forceinline__ int64_t slab_step(const Dims& d, int t) {
 // axis is a compile-time constant, so components and signs are too.
 template <int AX, typename T>
 __device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int i, int j, int k, T yc0, T yc1, T yb0,
-                                            T yb1, T e[3]) {
+                                           T yb1, T e[3]) {
   constexpr int C0 = AX == 0 ? 1 : 0, C1 = AX == 2 ? 1 : 2;
   constexpr int S0 = 3 - AX - C0, S1 = 3 - AX - C1;
   // phi_H = psi_{C+1} - psi_{C+2} enters H' with a minus sign
@@ -1003,23 +1008,21 @@ __device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int i, int j, i
   const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
 #pragma nounroll
   for (int side = 0; side < 2; ++side) {
-    const int f = a.face_of[AX][side];
-    if (f < 0) continue;
-    const FaceDev& fd = a.faces[f];
+    const AdjSide<T>& fd = a.side[AX * 2 + side];
     const int t = fd.t;
     const int l = side ? g - (n - t) : g;
-    if (l < 0 || l > t) continue;  // neither this cell nor the one back is in the slab
-    const gptr<const T> pin = gp(reinterpret_cast<const T*>(a.afaces[f].psiH[a.parity]));
-    const gptr<T> pout = gp(reinterpret_cast<T*>(a.afaces[f].psiH[1 - a.parity]));
+    if (t == 0 || l < 0 || l > t) continue;  // neither this cell nor the one back is in the slab
+    const gptr<const T> pin = gp(fd.pin);
+    const gptr<T> pout = gp(fd.pout);
     const int64_t off = slab_off<AX>(d, t, i, j, k, l), slab = fd.slab;
     if (l < t) {
       const T w0 = pin[off] + sg0 * yc0;
       const T w1 = pin[slab + off] + sg1 * yc1;
-      const T b = (T)gp(fd.bH)[l];
+      const T b = (T)gp(fd.b)[l];
       pout[off] = b * w0;
       pout[slab + off] = b * w1;
       if (l <= t - 2) {  // psi'(l) holds c(l) (E(l+1) - E(l))
-        const T c = (T)gp(fd.cH)[l];
+        const T c = (T)gp(fd.c)[l];
         e[S0] = e[S0] - c * w0;
         e[S1] = e[S1] - c * w1;
       }
@@ -1028,7 +1031,7 @@ __device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int i, int j, i
       const int64_t ob = off - slab_step<AX>(d, t);
       const T w0 = pin[ob] + sg0 * yb0;
       const T w1 = pin[slab + ob] + sg1 * yb1;
-      const T c = (T)gp(fd.cH)[l - 1];
+      const T c = (T)gp(fd.c)[l - 1];
       e[S0] = e[S0] + c * w0;
       e[S1] = e[S1] + c * w1;
     }
@@ -1037,9 +1040,11 @@ __device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int i, int j, i
 
 // The faces normal to AX at one cell of k_adj_E (see there): the face's own
 // psi transpose and the difference terms of this cell and of the face cell
-// one step ahead along AX.
+// one step ahead along AX. lc0, lc1 are lamE of the face's components C0, C1
+// at this cell, la0, la1 at the cell ahead (already loaded by the caller).
 template <int AX, typename T>
-__device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int i, int j, int k, int64_t cell, T h[3]) {
+__device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int i, int j, int k, int64_t cell, T lc0, T lc1,
+                                           T la0, T la1, T h[3]) {
   constexpr int C0 = AX == 0 ? 1 : 0, C1 = AX == 2 ? 1 : 2;
   constexpr int S0 = 3 - AX - C0, S1 = 3 - AX - C1;
   // phi_E = psi_{C+1} - psi_{C+2} enters E' with a plus sign
@@ -1050,35 +1055,33 @@ __device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int i, int j, i
   const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
   const int64_t st = AX == 0 ? (int64_t)d.ny * d.nz : (AX == 1 ? (int64_t)d.nz : 1);
   const T sc = a.sc;
-  auto KL = [&](int c, int64_t x) { return (sc * a.inv_eps[c * cs + x]) * a.lamE[c * cs + x]; };
+  auto K = [&](int c, int64_t x) { return sc * a.inv_eps[c * cs + x]; };
 #pragma nounroll
   for (int side = 0; side < 2; ++side) {
-    const int f = a.face_of[AX][side];
-    if (f < 0) continue;
-    const FaceDev& fd = a.faces[f];
+    const AdjSide<T>& fd = a.side[AX * 2 + side];
     const int t = fd.t;
     const int l = side ? g - (n - t) : g;
-    if (l < -1 || l >= t) continue;  // neither this cell nor the one ahead is in the slab
-    const gptr<const T> pin = gp(reinterpret_cast<const T*>(a.afaces[f].psiE[a.parity]));
-    const gptr<T> pout = gp(reinterpret_cast<T*>(a.afaces[f].psiE[1 - a.parity]));
+    if (t == 0 || l < -1 || l >= t) continue;  // neither this cell nor the one ahead is in the slab
+    const gptr<const T> pin = gp(fd.pin);
+    const gptr<T> pout = gp(fd.pout);
     const int64_t off = slab_off<AX>(d, t, i, j, k, l), slab = fd.slab;
     if (l >= 0) {
-      const T w0 = pin[off] + sg0 * KL(C0, cell);
-      const T w1 = pin[slab + off] + sg1 * KL(C1, cell);
-      const T b = (T)gp(fd.bE)[l];
+      const T w0 = pin[off] + sg0 * (K(C0, cell) * lc0);
+      const T w1 = pin[slab + off] + sg1 * (K(C1, cell) * lc1);
+      const T b = (T)gp(fd.b)[l];
       pout[off] = b * w0;
       pout[slab + off] = b * w1;
       if (l >= 1) {  // psi'(l) holds c(l) (H(l) - H(l-1))
-        const T c = (T)gp(fd.cE)[l];
+        const T c = (T)gp(fd.c)[l];
         h[S0] = h[S0] + c * w0;
         h[S1] = h[S1] + c * w1;
       }
     }
     if (l + 1 >= 1 && l + 1 <= t - 1 && g + 1 < n) {  // ... and the H(l-1) of slab cell l+1
       const int64_t ou = off + slab_step<AX>(d, t);
-      const T w0 = pin[ou] + sg0 * KL(C0, cell + st);
-      const T w1 = pin[slab + ou] + sg1 * KL(C1, cell + st);
-      const T c = (T)gp(fd.cE)[l + 1];
+      const T w0 = pin[ou] + sg0 * (K(C0, cell + st) * la0);
+      const T w1 = pin[slab + ou] + sg1 * (K(C1, cell + st) * la1);
+      const T c = (T)gp(fd.c)[l + 1];
       h[S0] = h[S0] - c * w0;
       h[S1] = h[S1] - c * w1;
     }
@@ -1196,17 +1199,24 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
   const int64_t zp = k < d.nz - 1 ? 1 : 0, yp = j < d.ny - 1 ? sy : 0;
   const T zero = T(0), sc = a.sc;
-  auto Y = [&](int c, int64_t x) { return a.gcoef[c * cs + x] * a.lamE[c * cs + x]; };
+  auto L = [&](int c, int64_t x) { return a.lamE[c * cs + x]; };
+  auto G = [&](int c, int64_t x) { return a.gcoef[c * cs + x]; };
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   T yz_next = zero, yy_next = zero;  // Y at plane i+1
+  T lz_next = zero, ly_next = zero;  // lamE at plane i+1
   if (i1 < d.nx) {
-    yz_next = Y(2, cell + sx);
-    yy_next = Y(1, cell + sx);
+    lz_next = L(2, cell + sx);
+    ly_next = L(1, cell + sx);
+    yz_next = G(2, cell + sx) * lz_next;
+    yy_next = G(1, cell + sx) * ly_next;
   }
   for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
-    const T y0 = Y(0, cell), y1 = Y(1, cell), y2 = Y(2, cell);
-    const T yy_kp = Y(1, cell + zp), yx_kp = Y(0, cell + zp);
-    const T yz_jp = Y(2, cell + yp), yx_jp = Y(0, cell + yp);
+    const T le0 = L(0, cell), le1 = L(1, cell), le2 = L(2, cell);
+    const T lk0 = L(0, cell + zp), lk1 = L(1, cell + zp);
+    const T lj0 = L(0, cell + yp), lj2 = L(2, cell + yp);
+    const T y0 = G(0, cell) * le0, y1 = G(1, cell) * le1, y2 = G(2, cell) * le2;
+    const T yy_kp = G(1, cell + zp) * lk1, yx_kp = G(0, cell + zp) * lk0;
+    const T yz_jp = G(2, cell + yp) * lj2, yx_jp = G(0, cell + yp) * lj0;
     const T l0 = a.lamH[cell], l1 = a.lamH[cs + cell], l2 = a.lamH[2 * cs + cell];
     const bool km = k >= 1, kp = k <= d.nz - 2, jm = j >= 1, jp = j <= d.ny - 2, im = i >= 1, ip = i <= d.nx - 2;
     const T t0 = ((km ? y1 : zero) - (kp ? yy_kp : zero)) - ((jm ? y2 : zero) - (jp ? yz_jp : zero));
@@ -1216,9 +1226,9 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
     if (act) {
       {
         T h[3] = {h0, h1, h2};
-        adj_faces_E<0>(a, i, j, k, cell, h);
-        adj_faces_E<1>(a, i, j, k, cell, h);
-        adj_faces_E<2>(a, i, j, k, cell, h);
+        adj_faces_E<0>(a, i, j, k, cell, le1, le2, ly_next, lz_next, h);
+        adj_faces_E<1>(a, i, j, k, cell, le0, le2, lj0, lj2, h);
+        adj_faces_E<2>(a, i, j, k, cell, le0, le1, lk0, lk1, h);
         h0 = h[0];
         h1 = h[1];
         h2 = h[2];
@@ -1229,13 +1239,16 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
       const int64_t pd = box_index(a.db, i, j, k);
       if (pd >= 0) {
         const int64_t nd = box_cells(a.db);
+        const T le[3] = {le0, le1, le2};
 #pragma unroll
         for (int c = 0; c < 3; ++c)
-          a.grad[c * nd + pd] += (double)sc * (double)a.lamE[c * cs + cell] * (double)a.curl[c * nd + pd];
+          a.grad[c * nd + pd] += (double)sc * (double)le[c] * (double)a.curl[c * nd + pd];
       }
     }
     yz_next = y2;
     yy_next = y1;
+    lz_next = le2;
+    ly_next = le1;
   }
 }
 

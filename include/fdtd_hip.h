@@ -250,6 +250,12 @@ int fdtd_run_recorded(FdtdAdjoint* adj, int64_t q0, int64_t nsteps, void* curl_r
 int fdtd_adjoint_run(FdtdAdjoint* adj, int64_t nsteps, const void* curl_records, const void* plane_records,
                      void* stream);
 
+/* x planes per workgroup of the transposed kernels (0, the default: the
+ * plan's choice for the main kernels). A tuning knob with no counterpart in
+ * the reference: when the transposed steps run beside a recomputation on
+ * another stream, fewer and longer workgroups leave room for it. */
+int fdtd_adjoint_set_launch(FdtdAdjoint* adj, int cx);
+
 /* Measurement hook, no counterpart in the reference (used by bench.py to
  * price the dominant kernels against the HBM roofline): launch the fused main
  * kernels of a step, E then H, `reps` times (at most 4096) on `stream` with a

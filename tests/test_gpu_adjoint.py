@@ -134,3 +134,32 @@ def test_c5_float32_gradient_error_against_float64():
     eg = rel(g32.cpu().numpy(), g64.cpu().numpy())
     _say(f"[gpu] adjoint c5 float32 vs float64: J rel {eJ:.2e}, grad rel L2 {eg:.2e}")
     assert eJ <= 1e-4 and eg <= 1e-3
+
+
+def test_transposed_launch_shape_leaves_gradient_unchanged():
+    """fdtd_adjoint_set_launch: x planes per workgroup of the transposed
+    kernels change the schedule only, not the arithmetic; negative values are
+    refused."""
+    from fdtd_amd import _native
+
+    sc = A.TINY_ABSORBER
+    prob, perm = _problem(sc, checkpoint="record")
+    J0, g0 = prob.gradient(torch.from_numpy(perm))
+    orig = _native.Adjoint.set_launch
+    grads = []
+    try:
+        for cx in (1, 3, 1000):
+            _native.Adjoint.set_launch = lambda self, _cx, cx=cx: orig(self, cx)
+            J, g = prob.gradient(torch.from_numpy(perm))
+            assert J == J0
+            grads.append(g.cpu().numpy())
+    finally:
+        _native.Adjoint.set_launch = orig
+    for g in grads:
+        assert np.array_equal(g, g0.cpu().numpy())
+    adj = _native.Adjoint(prob._plan(), *prob._coefficients(), torch.zeros_like(prob.grid._E),
+                          torch.zeros_like(prob.grid._H),
+                          torch.zeros(_native.Adjoint.psi_size(prob._plan()), dtype=prob.grid.dtype, device="cuda"),
+                          prob.design_box, prob.box, torch.zeros((3, prob.nd), dtype=torch.float64, device="cuda"))
+    with pytest.raises(RuntimeError, match="cx"):
+        adj.set_launch(-1)
