@@ -442,7 +442,9 @@ def main():
     if not args.no_cpu_baseline:
         base = cpu_baseline(n, pml, args.cpu_steps)
 
-    dt = "f64" if args.dtype == "float64" else "f32"
+    # arithmetic type; the compensated storage (float + 8 bits) computes in float64
+    dt = "f32" if args.dtype == "float32" else "f64"
+    storage = {"float64": "f64", "float32": "f32", "float32c": "f32+8 compensated"}[args.dtype]
     out = {
         "metric": METRIC,
         "value": round(main["mcells"], 1),
@@ -456,8 +458,8 @@ def main():
         "vs_baseline": None,
         "dtype": dt,
         "data": "synthetic",
-        "config": {"workload": f"{n}^3 grid, {pml}-cell PML x6, 24 dielectric spheres, pulsed point source, {dt}",
-                   "grid": [n, n, n], "pml": pml, "parallelism": "single GPU"},
+        "config": {"workload": f"{n}^3 grid, {pml}-cell PML x6, 24 dielectric spheres, pulsed point source, {storage}",
+                   "grid": [n, n, n], "pml": pml, "storage": storage, "parallelism": "single GPU"},
         "step_hbm_roofline_frac": round(main["step_frac"], 4),
         "copy_GBps": round(copy_gbs, 1),
         "roofline": main["roofline"],

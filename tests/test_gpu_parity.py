@@ -13,6 +13,7 @@ also demand bitwise equality.
 """
 
 import math
+import os
 
 import numpy as np
 import pytest
@@ -110,6 +111,36 @@ def test_float64_random_scenarios_match_oracle(seed):
     for d, o in zip(dets, odets):
         _assert_same(_np(d.trace("E")).reshape(np.array(o.E).shape), np.array(o.E), "detector E", bitwise)
         _assert_same(_np(d.trace("H")).reshape(np.array(o.H).shape), np.array(o.H), "detector H", bitwise)
+
+
+def test_save_data_writes_reference_readings(golden, tmp_path, monkeypatch):
+    """Grid.save_simulation() + save_data() (grid.py:407-439, 490-514) on the
+    G2 fixture run: fdtd_output/fdtd_output_<time> (<name>)/detector_readings.npz
+    with keys "<name> (E)" / "<name> (H)", line readings (steps, cells, 3) and
+    block readings (steps, nx, ny, nz, 3) -- the 3- and 5-axis shapes that
+    plot_detection and dB_map_2D require (visualization.py:358, 420) -- equal
+    to the reference's own readings bit for bit."""
+    g = golden("g2")
+    sc = scenarios.get_scenario("g2")
+    grid, comps = _build(sc)
+    scenarios.run_program(grid, sc)
+    monkeypatch.chdir(tmp_path)
+    with pytest.raises(Exception, match="Save location not initialized"):
+        grid.save_data()
+    folder = grid.save_simulation("readings")
+    assert os.path.dirname(folder) == str(tmp_path / "fdtd_output")
+    assert os.path.basename(folder).startswith("fdtd_output_") and folder.endswith(" (readings)")
+    grid.save_data()
+    dets = [(i, c) for i, c in enumerate(comps) if isinstance(c, (fdtd.LineDetector, fdtd.BlockDetector))]
+    assert len(dets) == 3
+    with np.load(os.path.join(folder, "detector_readings.npz")) as f:
+        assert sorted(f.files) == sorted(f"{c.name} ({F})" for _, c in dets for F in "EH")
+        for i, c in dets:
+            for F in "EH":
+                got = f[f"{c.name} ({F})"]
+                assert got.ndim == (3 if isinstance(c, fdtd.LineDetector) else 5)
+                assert got.shape[0] == grid.time_steps_passed
+                _assert_same(got, g[f"out/{i}/{F}"], f"saved {c.name} ({F})")
 
 
 def test_step_by_step_equals_run():
