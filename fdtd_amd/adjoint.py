@@ -20,10 +20,11 @@ steps (one forward pass plus one reverse pass). ``checkpoint="sqrt"`` keeps
 a copy of the whole state (E, H, every CPML psi) every K = ceil(sqrt(T))
 steps instead; the reverse pass restores each checkpoint, steps its segment
 forward again recording K steps, and runs the transposed steps over it (two
-forward passes plus one reverse pass, O(sqrt(T)) states of memory). The last
-RECORD_BUFFERS segments are recorded by the first forward pass into the
-record buffers the reverse pass uses, so they are neither checkpointed nor
-recomputed.
+forward passes plus one reverse pass, O(sqrt(T)) states of memory). There are
+2 to RECORD_BUFFERS record buffers of K steps (a quarter of the checkpoints'
+memory at most); the last segments, one per buffer, are recorded by the first
+forward pass into the buffers the reverse pass reads, so they are neither
+checkpointed nor recomputed.
 ``"auto"`` picks "record" when the records fit in a quarter of the free
 device memory.
 """
@@ -37,11 +38,14 @@ from . import _native
 from .boundaries import PML
 from .objects import AbsorbingObject, AnisotropicObject, Object
 
-# checkpointed mode: record buffers of K steps (the last RECORD_BUFFERS
-# segments are recorded by the forward pass, the others recomputed into them
-# in turn), and x planes per workgroup of the transposed kernels running
-# beside the recomputation (measured on MI355X at 128x128x64: 2 beats 1 and 4)
-RECORD_BUFFERS = 4
+# checkpointed mode: at most RECORD_BUFFERS record buffers of K steps (the
+# last ones' segments are recorded by the forward pass, the others recomputed
+# into them in turn; together they take at most a quarter of the checkpoints'
+# memory, and there are at least two), and x planes per workgroup of the
+# transposed kernels running beside the recomputation. Measured on MI355X at
+# 128x128x64, 2000 steps: 8 buffers 2.93x the forward, 6 2.96x, 4 2.98x;
+# 2 planes beat 1 and 4.
+RECORD_BUFFERS = 8
 SQRT_MODE_CX = 2
 
 
@@ -201,6 +205,15 @@ class IntensityGradient:
             t.zero_()
         self.grid.time_steps_passed = 0
 
+    def _record_buffers(self, K):
+        """Record buffers of the checkpointed mode: up to RECORD_BUFFERS, at
+        least 2, together at most a quarter of the checkpoints' memory."""
+        esz = self.grid._E.element_size()
+        per_buffer = K * 3 * (self.nd + self.np) * esz
+        checkpoint = sum(t.numel() * t.element_size() for t in self._state())
+        nseg = -(-self.steps // K)
+        return max(2, min(RECORD_BUFFERS, (nseg * checkpoint) // (4 * per_buffer)))
+
     def _mode(self):
         if self.checkpoint != "auto":
             return self.checkpoint
@@ -289,12 +302,12 @@ class IntensityGradient:
         psi = torch.zeros(_native.Adjoint.psi_size(plan), dtype=dt, device=dev)
         grad = torch.zeros((3, self.nd), dtype=torch.float64, device=dev)
         adj = _native.Adjoint(plan, gcoef, dcoef, lamE, lamH, psi, self.design_box, self.box, grad)
-        # in the checkpointed mode RECORD_BUFFERS buffers of K steps: the
+        # in the checkpointed mode several buffers of K steps: the
         # recomputation of earlier segments fills some while the transposed
         # steps of a later one read another; fewer and longer workgroups of
         # the transposed kernels leave room for the recomputation beside them
         sqrt_grad = gradient and mode == "sqrt"
-        nbuf = RECORD_BUFFERS if sqrt_grad else 1
+        nbuf = self._record_buffers(K) if sqrt_grad else 1
         adj.set_launch(SQRT_MODE_CX if sqrt_grad else 0)
         curl = [torch.empty((K, 3, self.nd), dtype=dt, device=dev) for _ in range(nbuf)]
         plane = [torch.empty((K, 3, self.np), dtype=dt, device=dev) for _ in range(nbuf)]
@@ -315,7 +328,7 @@ class IntensityGradient:
             adj.run_recorded(q, m, curl[b] if keep else None, plane[b], stream)
             J += plane[b][:m].to(torch.float64).square().sum()
         g.time_steps_passed = T
-        self.last_info = {"mode": mode, "segment": K, "checkpoints": len(checkpoints)}
+        self.last_info = {"mode": mode, "segment": K, "checkpoints": len(checkpoints), "record_buffers": nbuf}
         if not gradient:
             return float(J), None
         if mode == "record":
