@@ -266,13 +266,10 @@ class IntensityGradient:
                 recorded[s] = torch.cuda.Event()
                 recorded[s].record(side)
 
-        last = len(segments) - 1
-        pending = len(checkpoints) - 1  # the next segment to recompute, downwards
-        for s in range(last, -1, -1):
-            # buffer (s' % nbuf) was last read by segment s' + nbuf >= s + 1
-            while pending >= 0 and pending >= s + 1 - nbuf:
-                recompute(pending)
-                pending -= 1
+        for op, s in reverse_schedule(len(segments), nbuf):
+            if op == "recompute":
+                recompute(s)
+                continue
             b = s % nbuf
             if recorded[s] is not None:
                 cur.wait_event(recorded[s])
@@ -342,6 +339,26 @@ class IntensityGradient:
         # inv = 1/perm: dJ/dperm = -dJ/dinv / perm^2, summed over broadcast axes
         g_perm_b = -g_inv / perm_b.square()
         return float(J), _reduce_to(g_perm_b, permittivity)
+
+
+def reverse_schedule(nseg, nbuf):
+    """Host issue order of the checkpointed reverse sweep over nseg segments
+    with nbuf record buffers: ("recompute", s) and ("adjoint", s) pairs.
+    Segment s uses buffer s % nbuf; the last nbuf segments (recorded by the
+    forward pass) are not recomputed. Each segment's transposed steps come
+    after its recomputation, and a recomputation into buffer b comes after the
+    transposed steps of the segment that last held b (s + nbuf), so that a
+    wait issued with it sees them; within that bound recomputations run as
+    far ahead as the buffers allow."""
+    ops = []
+    pending = nseg - nbuf - 1  # the next segment to recompute, downwards
+    for s in range(nseg - 1, -1, -1):
+        # buffer (s' % nbuf) was last read by segment s' + nbuf >= s + 1
+        while pending >= 0 and pending >= s + 1 - nbuf:
+            ops.append(("recompute", pending))
+            pending -= 1
+        ops.append(("adjoint", s))
+    return ops
 
 
 def _reduce_to(g, permittivity):

@@ -247,3 +247,48 @@ def test_grid_energy_is_the_visualize_reduction(golden, name):
         grid.grid_energy(x=1, z=2)
     with pytest.raises(ValueError):
         grid.grid_energy()
+
+
+def _check_reverse_schedule(ops, nseg, nbuf):
+    """Replay a checkpointed reverse sweep's issue order against its record
+    buffers: every transposed segment finds its own record in its buffer,
+    no recomputation overwrites a buffer before the transposed steps of the
+    segment it holds were issued, the forward-recorded segments are never
+    recomputed and the others exactly once, transposed steps latest first."""
+    holder = {s % nbuf: s for s in range(max(0, nseg - nbuf), nseg)}
+    done, recomputed, order = set(), [], []
+    for op, s in ops:
+        b = s % nbuf
+        if op == "recompute":
+            assert holder.get(b) is None or holder[b] in done, f"buffer {b} of segment {holder.get(b)} overwritten"
+            holder[b] = s
+            recomputed.append(s)
+        else:
+            assert holder.get(b) == s, f"segment {s} transposed without its record"
+            done.add(s)
+            order.append(s)
+    assert order == list(range(nseg - 1, -1, -1))
+    assert sorted(recomputed) == list(range(max(0, nseg - nbuf)))
+
+
+def test_checkpointed_reverse_schedule():
+    """fdtd_amd.adjoint.reverse_schedule for 1-80 segments and 2-8 buffers;
+    a schedule that recomputes one segment further ahead than the buffers
+    allow is caught."""
+    from fdtd_amd.adjoint import reverse_schedule
+
+    for nseg in range(1, 81):
+        for nbuf in range(2, 9):
+            _check_reverse_schedule(reverse_schedule(nseg, nbuf), nseg, nbuf)
+
+    def too_far_ahead(nseg, nbuf):
+        ops, pending = [], nseg - nbuf - 1
+        for s in range(nseg - 1, -1, -1):
+            while pending >= 0 and pending >= s - nbuf:
+                ops.append(("recompute", pending))
+                pending -= 1
+            ops.append(("adjoint", s))
+        return ops
+
+    with pytest.raises(AssertionError, match="overwritten"):
+        _check_reverse_schedule(too_far_ahead(20, 4), 20, 4)
