@@ -292,3 +292,26 @@ def test_checkpointed_reverse_schedule():
 
     with pytest.raises(AssertionError, match="overwritten"):
         _check_reverse_schedule(too_far_ahead(20, 4), 20, 4)
+
+
+def test_checkpointed_record_buffer_count():
+    """IntensityGradient._record_buffers: between 2 and RECORD_BUFFERS buffers
+    of K steps, together at most a quarter of the checkpoints' memory."""
+    from types import SimpleNamespace
+
+    from fdtd_amd.adjoint import RECORD_BUFFERS, IntensityGradient
+
+    E = torch.zeros((3, 128, 128, 64), dtype=torch.float32)
+    psi = [torch.zeros(2 * 12 * 128 * 64, dtype=torch.float32) for _ in range(12)]
+
+    def count(nd, steps=2000, K=45):
+        fake = SimpleNamespace(grid=SimpleNamespace(_E=E), steps=steps, nd=nd, np=256,
+                               _state=lambda: [E, E] + psi)
+        return IntensityGradient._record_buffers(fake, K)
+
+    assert count(64 * 64 * 16) == RECORD_BUFFERS        # the differentiation workload: small records
+    assert count(128 * 128 * 64) == 2                    # a whole-grid design: records as large as a state
+    n = count(128 * 128 * 16)
+    assert 2 <= n <= RECORD_BUFFERS
+    state = sum(t.numel() * 4 for t in [E, E] + psi)
+    assert n * 45 * 3 * (128 * 128 * 16 + 256) * 4 <= 45 * state / 4
