@@ -210,7 +210,7 @@ PlainField<float, float> make_field(const FdtdPlan* p, bool H) {
 }
 template <>
 CompField make_field(const FdtdPlan* p, bool H) {
-  return {reinterpret_cast<float*>(H ? p->H : p->E), reinterpret_cast<float*>(H ? p->H_lo : p->E_lo), p->d.cs};
+  return {reinterpret_cast<float*>(H ? p->H : p->E), p->d.cs};
 }
 template <>
 PlainField<cplx<double>, cplx<double>> make_field(const FdtdPlan* p, bool H) {
@@ -707,6 +707,8 @@ int fdtd_plan_set_compensation(FdtdPlan* p, void* E_lo, void* H_lo) {
   if (!p) return fail(-1, "null plan");
   if (p->dtype != FDTD_F32C) return fail(-1, "compensation arrays belong to FDTD_F32C plans");
   if (!E_lo || !H_lo) return fail(-1, "null compensation array");
+  if (E_lo != static_cast<char*>(p->E) + 3 * sizeof(float) || H_lo != static_cast<char*>(p->H) + 3 * sizeof(float))
+    return fail(-1, "FDTD_F32C compensation words must be the 4th float of the E and H records");
   p->E_lo = E_lo;
   p->H_lo = H_lo;
   p->finalized = false;

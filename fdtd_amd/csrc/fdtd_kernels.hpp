@@ -227,62 +227,80 @@ __device__ __forceinline__ float comp_hi(double v, uint32_t& q) {
   return (float)__longlong_as_double((long long)(u & ~((1ull << 29) - 1)));
 }
 
-// The floats are (3, N) like a plain field; the three compensation bytes of
-// a cell share one 32-bit word (byte c for component c), so a cell's
-// components cost one extra 4-byte access, not three byte accesses. The words
-// are accessed as float bit patterns (moved, never computed with): stores of
+// A cell is one 16-byte record, (N, 4) floats: its x, y, z floats and a
+// 32-bit word holding the three compensation bytes (byte c for component c),
+// so a cell's three components and their compensation come in one 16-byte
+// access (four 4-byte accesses with the floats (3, N) and the words apart),
+// with no padding. Loads are 16-byte vectors; stores are four float stores
+// (merged into one 16-byte store by the backend): a store of the vector type
+// may alias any table, with the effect described next. The words are accessed
+// as float bit patterns (moved, never computed with): stores of
 // an integer type could alias the integer fields of the face and object
 // tables that the same loops read, and type-based alias analysis would then
 // turn those uniform table reads into vector loads that wait for every
 // outstanding store, once per face and cell.
+// 16-byte vector of four floats (GCC vector extension: element access by [])
+typedef float f32x4 __attribute__((vector_size(16)));
+
 struct CompField {
   using R = double;
   using Sample = float;
-  float* hi;  // (3, N)
-  float* lo;  // (N) words
+  float* rec;  // (N, 4): x, y, z rounded to float, then the compensation word
   int64_t cs;
   __device__ __forceinline__ int comp_of(int64_t i) const { return i >= 2 * cs ? 2 : (i >= cs ? 1 : 0); }
-  __device__ __forceinline__ uint32_t word(int64_t cell) const { return __float_as_uint(gp(lo)[cell]); }
+  template <bool NT>
+  __device__ __forceinline__ f32x4 record(int64_t cell) const {
+    return ld_stream<NT>(reinterpret_cast<const f32x4*>(rec) + cell);
+  }
+  __device__ __forceinline__ uint32_t word(int64_t cell) const { return __float_as_uint(gp(rec)[4 * cell + 3]); }
   __device__ __forceinline__ void set_byte(int64_t cell, int c, uint32_t q) const {
-    gp(lo)[cell] = __uint_as_float((word(cell) & ~(0xffu << (8 * c))) | (q << (8 * c)));
+    gp(rec)[4 * cell + 3] = __uint_as_float((word(cell) & ~(0xffu << (8 * c))) | (q << (8 * c)));
   }
   __device__ __forceinline__ double get(int64_t i) const {
     const int c = comp_of(i);
-    return comp_decode(gp(hi)[i], (word(i - c * cs) >> (8 * c)) & 0xffu);
+    const int64_t cell = i - c * cs;
+    return comp_decode(gp(rec)[4 * cell + c], (word(cell) >> (8 * c)) & 0xffu);
   }
   __device__ __forceinline__ void put(int64_t i, double v) const {
     const int c = comp_of(i);
+    const int64_t cell = i - c * cs;
     uint32_t q;
-    gp(hi)[i] = comp_hi(v, q);
-    set_byte(i - c * cs, c, q);
+    gp(rec)[4 * cell + c] = comp_hi(v, q);
+    set_byte(cell, c, q);
   }
   __device__ __forceinline__ void copy(int64_t dst, int64_t src) const {
     const int cd = comp_of(dst), cs_ = comp_of(src);
-    gp(hi)[dst] = gp(hi)[src];
-    set_byte(dst - cd * cs, cd, (word(src - cs_ * cs) >> (8 * cs_)) & 0xffu);
+    const int64_t celld = dst - cd * cs, cells = src - cs_ * cs;
+    gp(rec)[4 * celld + cd] = gp(rec)[4 * cells + cs_];
+    set_byte(celld, cd, (word(cells) >> (8 * cs_)) & 0xffu);
   }
   // detectors sample the value rounded to float
   __device__ __forceinline__ float sample(int64_t i) const { return (float)get(i); }
   template <int C>
   __device__ __forceinline__ double get_c(int64_t cell) const {
-    return comp_decode(gp(hi)[C * cs + cell], (word(cell) >> (8 * C)) & 0xffu);
+    const f32x4 r = record<false>(cell);
+    return comp_decode(r[C], (__float_as_uint(r[3]) >> (8 * C)) & 0xffu);
   }
   template <bool NT>
   __device__ __forceinline__ void get3_s(int64_t cell, double v[3]) const {
-    const uint32_t w = __float_as_uint(ld_stream<NT>(lo + cell));
+    const f32x4 r = record<NT>(cell);
+    const uint32_t w = __float_as_uint(r[3]);
 #pragma unroll
-    for (int c = 0; c < 3; ++c) v[c] = comp_decode(ld_stream<NT>(hi + c * cs + cell), (w >> (8 * c)) & 0xffu);
+    for (int c = 0; c < 3; ++c) v[c] = comp_decode(r[c], (w >> (8 * c)) & 0xffu);
   }
   template <bool NT>
   __device__ __forceinline__ void put3_s(int64_t cell, const double v[3]) const {
+    f32x4 r;
     uint32_t w = 0;
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
       uint32_t q;
-      st_stream<NT>(hi + c * cs + cell, comp_hi(v[c], q));
+      r[c] = comp_hi(v[c], q);
       w |= q << (8 * c);
     }
-    st_stream<NT>(lo + cell, __uint_as_float(w));
+    r[3] = __uint_as_float(w);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) st_stream<NT>(rec + 4 * cell + c, (float)r[c]);
   }
 };
 

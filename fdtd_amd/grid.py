@@ -97,9 +97,17 @@ class Grid:
             # grid.E/H[...] changes the float part only: the byte left there
             # lies below the new float's last bit, so the written value is
             # kept to float32 precision and every other cell keeps 32 bits.
-            # one 32-bit word per cell: byte c holds component c's bits
-            self._E_lo = torch.zeros(soa[1:], dtype=torch.int32, device=dev)
-            self._H_lo = torch.zeros(soa[1:], dtype=torch.int32, device=dev)
+            # One 16-byte record per cell, (Nx, Ny, Nz, 4) float32: x, y, z and
+            # a 32-bit word whose byte c holds component c's bits. _E/_H are
+            # the (3, Nx, Ny, Nz) float views of the records, _E_lo/_H_lo the
+            # words, so that the kernels read a cell with one 16-byte access.
+            rec = (self.Nx, self.Ny, self.Nz, 4)
+            self._E_rec = torch.zeros(rec, dtype=torch.float32, device=dev)
+            self._H_rec = torch.zeros(rec, dtype=torch.float32, device=dev)
+            self._E = self._E_rec[..., :3].permute(3, 0, 1, 2)
+            self._H = self._H_rec[..., :3].permute(3, 0, 1, 2)
+            self._E_lo = self._E_rec[..., 3].view(torch.int32)
+            self._H_lo = self._H_rec[..., 3].view(torch.int32)
         # materials read through a <=256-entry float64 palette: "auto" uses one
         # in the compensated mode when the grid has few distinct materials
         self.material_palette = "auto"

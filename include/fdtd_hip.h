@@ -18,8 +18,9 @@
  *     (nx, ny, nz, 3) or (nx, ny, nz, 3, 3).
  *   - dtype: FDTD_F64 or FDTD_F32 for every floating array of the plan, or
  *     FDTD_F32C: E and H stored as float32 plus an 8-bit compensation term per
- *     value (fdtd_plan_set_compensation), arithmetic in float64, materials,
- *     object arrays and CPML state in float64, detector traces in float32.
+ *     value, one 16-byte record per cell (fdtd_plan_set_compensation),
+ *     arithmetic in float64, materials, object arrays and CPML state in
+ *     float64, detector traces in float32.
  *   - FDTD_C128 / FDTD_C64: the complex-field path of the reference
  *     (fdtd/grid.py:353-356): E, H, CPML state and object inverse
  *     permittivities complex (interleaved re, im; complex128 / complex64),
@@ -55,7 +56,8 @@ const char* fdtd_last_error(void);
 
 /* New plan over a grid of nx*ny*nz cells.
  * Replaces the state set up by Grid.__init__ (fdtd/grid.py:90-169):
- * E, H (3,nx,ny,nz) read-write; inv_eps, inv_mu (3,nx,ny,nz) read-only,
+ * E, H (3,nx,ny,nz) read-write (FDTD_F32C: (nx,ny,nz,4) records, see
+ * fdtd_plan_set_compensation); inv_eps, inv_mu (3,nx,ny,nz) read-only,
  * read every step so that writes to the grid materials take effect. */
 int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number,
                      void* E, void* H, const void* inv_eps, const void* inv_mu,
@@ -63,11 +65,13 @@ int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number,
 
 int fdtd_plan_destroy(FdtdPlan* plan);
 
-/* FDTD_F32C only, required: the compensation terms of E and H, one uint32
- * per cell each, (nx, ny, nz); byte c of a cell's word belongs to component
- * c. A value is the float E (its top 24 significant bits) extended by the 8
- * mantissa bits that follow, held in its byte: 32 significant bits in all; a
- * zero byte leaves the float's value unchanged. */
+/* FDTD_F32C only, required. E and H of such a plan are (nx, ny, nz, 4)
+ * float32 records, z fastest: x, y, z of the field at the cell, then a uint32
+ * word whose byte c holds component c's compensation term (E_lo / H_lo: the
+ * word of the first cell, i.e. E + 3 floats / H + 3 floats; other addresses
+ * are refused). A value is the float (its top 24 significant bits) extended by
+ * the 8 mantissa bits that follow, held in its byte: 32 significant bits in
+ * all; a zero byte leaves the float's value unchanged. */
 int fdtd_plan_set_compensation(FdtdPlan* plan, void* E_lo, void* H_lo);
 
 /* Optional, any dtype: read the grid materials through a palette instead of
