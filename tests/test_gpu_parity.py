@@ -319,6 +319,24 @@ def test_compensated_float32_error_against_float64_reference(golden, name):
             assert rel_l2(_np(c.trace("E")), g[f"out/{i}/E"]) < 1e-6
 
 
+def test_compensated_words_are_the_record_slots():
+    """hip.float32c fields are (Nx, Ny, Nz, 4) float32 records: the float32 E/H
+    views are [..., :3] of them and the compensation words [..., 3];
+    fdtd_plan_set_compensation refuses words anywhere else."""
+    from fdtd_amd import _native
+
+    fdtd.set_backend("hip.float32c")
+    grid = fdtd.Grid((6, 5, 4), grid_spacing=50e-9, courant_number=0.5)
+    assert grid._E_rec.shape == (6, 5, 4, 4) and grid._E_rec.is_contiguous()
+    assert grid.E.data_ptr() == grid._E_rec.data_ptr() and grid.E.stride() == (80, 16, 4, 1)
+    assert grid._E_lo.data_ptr() == grid._E_rec.data_ptr() + 12
+    plan = _native.Plan(_native.F32C, (6, 5, 4), grid.courant_number, grid._E, grid._H, grid._inv_eps, grid._inv_mu)
+    with pytest.raises(RuntimeError, match="4th float"):
+        plan.set_compensation(torch.zeros((6, 5, 4), dtype=torch.int32, device=grid._E.device), grid._H_lo)
+    plan.set_compensation(grid._E_lo, grid._H_lo)
+    fdtd.set_backend("hip.float64")
+
+
 def test_frequency_routines_match_reference(golden):
     """FrequencyRoutines on the device (torch.fft, complex128) after the
     soft-source run, against what the reference's routines computed on its
