@@ -315,3 +315,37 @@ def test_checkpointed_record_buffer_count():
     assert 2 <= n <= RECORD_BUFFERS
     state = sum(t.numel() * 4 for t in [E, E] + psi)
     assert n * 45 * 3 * (128 * 128 * 16 + 256) * 4 <= 45 * state / 4
+
+
+def test_compensated_records_and_host_writes():
+    """hip.float32c storage (host arrays): one (Nx, Ny, Nz, 4) float32 record
+    per cell; a whole-field assignment keeps 32 significant bits (the float
+    and the next 8 mantissa bits, rounded at bit 21 of the double), a write
+    into grid.E[...] changes the written floats only and leaves every other
+    cell's 40-bit value untouched."""
+    fdtd.set_backend("hip.float32c")
+    try:
+        grid = fdtd.Grid((5, 4, 3), grid_spacing=50e-9, courant_number=0.5)
+        assert grid._E_rec.shape == (5, 4, 3, 4)
+        rs = np.random.RandomState(7)
+        v = rs.uniform(-1, 1, size=(5, 4, 3, 3))
+        grid.E = torch.from_numpy(v)
+        # the 32-bit rounding of v, from its bit pattern (numpy, independent of the grid code)
+        bits = v.view(np.int64)
+        want = ((bits + (1 << 20)) & ~np.int64((1 << 21) - 1)).view(np.float64)
+        got = grid.full_precision("E").numpy()
+        assert np.array_equal(got, want)
+        assert np.abs(got - v).max() <= np.abs(v).max() * 2.0 ** -32
+        # the float part: the top 24 significant bits of that rounding (exact in float32)
+        hi = (want.view(np.int64) & ~np.int64((1 << 29) - 1)).view(np.float64)
+        assert np.array_equal(grid.E.numpy(), hi.astype(np.float32))
+        grid.E[1:3, :, 2, 0] = 0.25
+        after = grid.full_precision("E").numpy()
+        mask = np.zeros(v.shape, bool)
+        mask[1:3, :, 2, 0] = True
+        assert np.array_equal(after[~mask], want[~mask])
+        assert np.all(np.abs(after[mask] - 0.25) < 0.25 * 2.0 ** -23)
+        # the words are the records' 4th floats
+        assert grid._E_lo.data_ptr() == grid._E_rec.data_ptr() + 12
+    finally:
+        fdtd.set_backend("hip.float64")
