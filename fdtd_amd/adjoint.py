@@ -43,9 +43,9 @@ from .objects import AbsorbingObject, AnisotropicObject, Object
 # into them in turn; together they take at most a quarter of the checkpoints'
 # memory, and there are at least two), and x planes per workgroup of the
 # transposed kernels running beside the recomputation. Measured on MI355X at
-# 128x128x64, 2000 steps: 8 buffers 2.93x the forward, 6 2.96x, 4 2.98x;
-# 2 planes beat 1 and 4.
-RECORD_BUFFERS = 8
+# 128x128x64, 2000 steps: 10 buffers (the memory bound there) 2.89x the
+# forward, 8 2.94x, 6 2.96x, 4 2.98x; 2 planes beat 1 and 4.
+RECORD_BUFFERS = 16
 SQRT_MODE_CX = 2
 
 

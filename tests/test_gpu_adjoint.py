@@ -48,12 +48,17 @@ def _fp64():
     fdtd.set_backend("hip.float64")
 
 
-def test_g5_gradient_matches_reference_autograd(golden):
+def test_g5_gradient_matches_reference_autograd(golden, monkeypatch):
+    import fdtd_amd.adjoint as adjoint_mod
+
     f = golden("adjoint_g5")
     grads = {}
-    for mode in ("record", "sqrt"):
+    # "sqrt2": the checkpointed mode with two record buffers, so that all but
+    # the last two segments are recomputed
+    for mode in ("record", "sqrt", "sqrt2"):
         _say(f"[gpu] adjoint g5 {mode}")
-        prob, perm = _problem(A.G5, checkpoint=mode)
+        monkeypatch.setattr(adjoint_mod, "RECORD_BUFFERS", 2 if mode == "sqrt2" else 16)
+        prob, perm = _problem(A.G5, checkpoint=mode[:4] if mode != "record" else mode)
         assert np.array_equal(perm, f["permittivity"])
         J, g = prob.gradient(torch.from_numpy(perm))
         g = g.cpu().numpy()
@@ -67,6 +72,7 @@ def test_g5_gradient_matches_reference_autograd(golden):
         grads[mode] = g
     # the checkpointed reverse pass recomputes bit-identical records
     assert np.array_equal(grads["record"], grads["sqrt"])
+    assert np.array_equal(grads["record"], grads["sqrt2"])
 
 
 def test_autograd_function_backward_equals_gradient():

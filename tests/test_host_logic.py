@@ -309,7 +309,8 @@ def test_checkpointed_record_buffer_count():
                                _state=lambda: [E, E] + psi)
         return IntensityGradient._record_buffers(fake, K)
 
-    assert count(64 * 64 * 16) == RECORD_BUFFERS        # the differentiation workload: small records
+    n_small = count(64 * 64 * 16)                        # the differentiation workload: small records
+    assert 8 <= n_small <= RECORD_BUFFERS
     assert count(128 * 128 * 64) == 2                    # a whole-grid design: records as large as a state
     n = count(128 * 128 * 16)
     assert 2 <= n <= RECORD_BUFFERS
