@@ -18,6 +18,7 @@ numpy oracle on the same configuration, a bounded number of steps, one core.
 import argparse
 import json
 import os
+import platform
 import sys
 import time
 
@@ -94,7 +95,9 @@ def copy_bandwidth(torch, nbytes=1 << 30, reps=10):
 
 
 def cpu_baseline(n, pml, steps):
-    """The numpy oracle (single-threaded ufuncs) on the same workload."""
+    """The numpy oracle (single-threaded ufuncs) on the same workload: `steps`
+    timed steps after one untimed one; the value is the median step, the
+    spread the fastest and slowest."""
     from oracle import fdtd_oracle as orc
 
     sim = orc.Sim((n, n, n), grid_spacing=50e-9, permittivity=sphere_permittivity(n), courant_number=3 ** -0.5)
@@ -102,13 +105,24 @@ def cpu_baseline(n, pml, steps):
         sim.add("PML", k)
     sim.add("PointSource", (n // 2, n // 2, n // 2), period=1550e-9 / 299792458.0, pulse=True)
     sim.step()  # first step outside the timing (page faults of fresh arrays)
-    t0 = time.perf_counter()
+    per = []
     for _ in range(steps):
+        t0 = time.perf_counter()
         sim.step()
-    dt = time.perf_counter() - t0
-    return dict(value=n ** 3 * steps / dt / 1e6, unit="Mcells/s", cores=1, kind="port",
-                sample=f"{steps} steps of the {n}^3 workload (after 1 untimed step), numpy oracle, "
-                       f"{os.cpu_count()} host cores visible, ufuncs single-threaded")
+        per.append(time.perf_counter() - t0)
+    rate = lambda dt: n ** 3 / dt / 1e6
+    cpu = platform.processor() or platform.machine()
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                cpu = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(value=rate(float(np.median(per))), unit="Mcells/s", cores=1, kind="port",
+                spread={"min": round(rate(max(per)), 3), "max": round(rate(min(per)), 3), "steps": steps},
+                sample=f"{steps} steps of the {n}^3 workload (after 1 untimed step), median step; numpy oracle, "
+                       f"ufuncs single-threaded; {os.cpu_count()} host cores visible ({cpu})")
 
 
 TRAFFIC_PROFILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -128,23 +142,46 @@ def profiled_traffic(kernel, ctype, n, pml):
     return None
 
 
-def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps):
-    """Mcells/s over `steps` timed steps and the roofline of the dominant kernel."""
+WINDOWS = 3  # timed windows of K steps each; the median is reported
+
+
+def field_errors(got, ref):
+    """Error of a run's final fields against the float64 run of the same
+    workload (device tensors of every E and H value, float64): relative L2
+    norm, max-norm relative error (max |difference| / max |float64 value|)
+    and the pointwise relative error with a floor of 1e-3 max |float64 value|."""
+    d = (got - ref).abs()
+    mx = ref.abs().max()
+    return {"rel_l2": float((got - ref).norm() / ref.norm()), "max_norm_rel": float(d.max() / mx),
+            "pointwise_rel_floor_1e-3max": float((d / ref.abs().clamp_min(1e-3 * mx)).max())}
+
+
+def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps, keep_fields=False):
+    """Mcells/s over `steps` timed steps (median of WINDOWS windows) and the
+    roofline of the dominant kernel; with keep_fields, also the final E and H
+    (float64, decoded) for error figures."""
     fdtd.set_backend(f"hip.{dtype}")
     grid = build_grid(fdtd, n, pml)
     grid.run(warmup, progress_bar=False)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record(stream)
-    grid.run(steps, progress_bar=False)
-    ev1.record(stream)
-    torch.cuda.synchronize()
-    ms = ev0.elapsed_time(ev1)
+    windows = []
+    for _ in range(WINDOWS):
+        ev0 = torch.cuda.Event(enable_timing=True)
+        ev1 = torch.cuda.Event(enable_timing=True)
+        torch.cuda.synchronize()
+        ev0.record(stream)
+        grid.run(steps, progress_bar=False)
+        ev1.record(stream)
+        torch.cuda.synchronize()
+        windows.append(ev0.elapsed_time(ev1))
+    ms = float(np.median(windows))
     ms_step = ms / steps
     itemsize = 8 if dtype == "float64" else 4
     half_bytes = algorithmic_bytes_per_half_step(n, pml, itemsize, compensated=dtype == "float32c")
+    fields = None
+    if keep_fields:
+        fields = torch.cat([grid.full_precision("E").reshape(-1), grid.full_precision("H").reshape(-1)])
     # dominant kernel: the fused main kernels alone, HIP events on the launch stream
     plan = grid._ensure_plan()
     plan.bench_main_kernel(5, stream.cuda_stream)
@@ -152,17 +189,21 @@ def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps):
     per_launch = {"k_main_E": tE, "k_main_H": tH}
     dom = max(per_launch, key=per_launch.get)
     achieved = half_bytes / (per_launch[dom] / 1e3) / 1e9
+    ctype = {"float64": "double", "float32": "float"}.get(dtype, "compensated")
+    total_steps = warmup + WINDOWS * steps
     del grid, plan
     torch.cuda.empty_cache()
     return dict(
         mcells=n ** 3 * steps / (ms / 1e3) / 1e6,
         ms_step=ms_step,
+        windows_ms_per_step=[round(w / steps, 4) for w in windows],
+        steps_run=total_steps,
         step_frac=2 * half_bytes / (ms_step / 1e3) / (HBM_PEAK_GBS * 1e9),
-        roofline={"bound": "hbm", "kernel": f"{dom}<{ {'float64': 'double', 'float32': 'float'}.get(dtype, dtype)}>",
+        fields=fields,
+        roofline={"bound": "hbm", "kernel": f"{dom}<{ctype}>",
                   "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                   "frac": round(achieved / HBM_PEAK_GBS, 4),
-                  "traffic": profiled_traffic(dom, {"float64": "double", "float32": "float"}.get(dtype, "compensated"),
-                                              n, pml),
+                  "traffic": profiled_traffic(dom, ctype, n, pml),
                   "algorithmic_bytes_per_launch": half_bytes,
                   "avg_launch_ms": {k: round(v, 5) for k, v in per_launch.items()}})
 
@@ -184,25 +225,55 @@ def spd_inverse_block(shape, seed=7, lo=1.0, hi=9.0):
     return inv.reshape(tuple(shape) + (3, 3))
 
 
-def build_precision_grid(fdtd, shape=(512, 512, 384), pml=20):
-    """The precision configuration: 20-cell PML on all faces, an absorbing
-    slab (conductivity 1e4 S/m), an anisotropic block with a seeded SPD
-    permittivity (eigenvalues 1-9) and a continuous 1550 nm point source."""
+def precision_components(shape=(512, 512, 384), pml=20):
+    """Registration list of the precision configuration, (kind, key, params,
+    attributes set after registration): `pml`-cell PML on all faces, an
+    absorbing slab (conductivity 1e4 S/m, 24 cells thick at 512 cells along
+    x), an anisotropic block of one eighth of each edge with a seeded SPD
+    permittivity (eigenvalues 1-9) and a continuous 1550 nm point source at
+    the centre. The object sizes scale with the shape, so a small copy of the
+    configuration keeps its structure."""
+    nx, ny, nz = shape
+    comps = [("PML", k, {}, {}) for k in PML_KEYS(pml)]
+    x0, sx = int(0.65 * nx), max(2, round(24 * nx / 512))
+    comps.append(("AbsorbingObject", (slice(x0, x0 + sx), slice(pml, ny - pml), slice(pml, nz - pml)),
+                  dict(permittivity=2.25, conductivity=1e4), {}))
+    bx, by, bz = nx // 8, ny // 8, nz // 8
+    box = (slice(int(0.25 * nx), int(0.25 * nx) + bx), slice(int(0.4 * ny), int(0.4 * ny) + by),
+           slice(int(0.35 * nz), int(0.35 * nz) + bz))
+    comps.append(("AnisotropicObject", box, dict(permittivity=1.0),
+                  {"inverse_permittivity": spd_inverse_block((bx, by, bz))}))
+    comps.append(("PointSource", (nx // 2, ny // 2, nz // 2), dict(period=1550e-9 / 299792458.0), {}))
+    return comps
+
+
+PRECISION_COURANT = 0.99 * 3 ** -0.5
+
+
+def build_precision_grid(fdtd, shape=(512, 512, 384), pml=20, amplitude=1.0):
+    """The precision configuration on the fdtd_amd API (see
+    precision_components); `amplitude` scales the source."""
     import torch
 
-    nx, ny, nz = shape
-    grid = fdtd.Grid(shape, grid_spacing=50e-9, courant_number=0.99 * 3 ** -0.5)
-    for k in PML_KEYS(pml):
-        grid[k] = fdtd.PML()
-    grid[int(0.65 * nx):int(0.65 * nx) + 24, pml:ny - pml, pml:nz - pml] = fdtd.AbsorbingObject(
-        permittivity=2.25, conductivity=1e4)
-    box = (slice(int(0.25 * nx), int(0.25 * nx) + 64), slice(int(0.4 * ny), int(0.4 * ny) + 64),
-           slice(int(0.35 * nz), int(0.35 * nz) + 48))
-    aniso = fdtd.AnisotropicObject(permittivity=1.0)
-    grid[box] = aniso
-    aniso.inverse_permittivity = torch.from_numpy(spd_inverse_block((64, 64, 48)))
-    grid[nx // 2, ny // 2, nz // 2] = fdtd.PointSource(period=1550e-9 / 299792458.0)
+    grid = fdtd.Grid(shape, grid_spacing=50e-9, courant_number=PRECISION_COURANT)
+    for kind, key, params, attrs in precision_components(shape, pml):
+        if kind == "PointSource":
+            params = dict(params, amplitude=amplitude)
+        comp = getattr(fdtd, kind)(**params)
+        grid[key] = comp
+        for name, value in attrs.items():
+            setattr(comp, name, torch.from_numpy(value))
     return grid
+
+
+def build_precision_oracle(orc, shape, pml=20):
+    """The same configuration through the oracle's registration API."""
+    sim = orc.Sim(shape, 50e-9, 1.0, 1.0, PRECISION_COURANT)
+    for kind, key, params, attrs in precision_components(shape, pml):
+        comp = sim.add(kind, key, **params)
+        for name, value in attrs.items():
+            setattr(comp, name, value)
+    return sim
 
 
 def precision_form(args):
@@ -236,9 +307,7 @@ def precision_form(args):
         if ref is None:
             ref = fields
         else:
-            diff = fields - ref
-            run["rel_l2_error_vs_f64"] = float(diff.norm() / ref.norm())
-            run["max_rel_error_vs_f64"] = float(diff.abs().max() / ref.abs().max())
+            run["error_vs_f64"] = field_errors(fields, ref)
         out["runs"][name] = run
         del grid, fields
         torch.cuda.empty_cache()
@@ -306,28 +375,54 @@ def adjoint_measure(dtype="float32", checkpoints=("sqrt", "record"), reps=3):
     return out
 
 
-def batch_grids(fdtd, torch, count=64, seed=0, pml=10):
-    """The batching configuration: `count` grids with edges drawn uniformly
-    from 64-192 cells, each with a per-voxel permittivity of smoothed Gaussian
-    noise mapped to [1, 12] (noise on a grid 8x coarser, trilinearly
-    interpolated), 10-cell PML on all faces and a pulsed 1550 nm point source
-    at a random interior position."""
+def batch_specs(count=64, seed=0):
+    """The batching configuration's grids as (shape, permittivity, source
+    cell): edges drawn uniformly from 64-192 cells, a per-voxel permittivity
+    of smoothed Gaussian noise mapped to [1, 12] (noise on a grid 8x coarser,
+    trilinearly interpolated) and a random source position at least 10 cells
+    from every face."""
+    import torch
     import torch.nn.functional as Fn
 
     rng = np.random.default_rng(seed)
-    grids = []
+    out = []
     for _ in range(count):
         shape = tuple(int(v) for v in rng.integers(64, 193, size=3))
         coarse = torch.from_numpy(rng.normal(size=tuple(n // 8 + 2 for n in shape)))
         fine = Fn.interpolate(coarse[None, None], size=shape, mode="trilinear", align_corners=True)[0, 0]
         eps = 1.0 + 11.0 * (fine - fine.min()) / (fine.max() - fine.min())
-        grid = fdtd.Grid(shape, grid_spacing=50e-9, permittivity=eps[..., None], courant_number=3 ** -0.5)
-        for k in PML_KEYS(pml):
-            grid[k] = fdtd.PML()
-        p = [int(rng.integers(pml, n - pml)) for n in shape]
-        grid[p[0], p[1], p[2]] = fdtd.PointSource(period=1550e-9 / 299792458.0, pulse=True)
-        grids.append(grid)
-    return grids
+        p = tuple(int(rng.integers(BATCH_PML, n - BATCH_PML)) for n in shape)
+        out.append((shape, eps, p))
+    return out
+
+
+BATCH_PML = 10
+
+
+def batch_grid(fdtd, spec):
+    """One grid of the batching configuration: 10-cell PML on all faces and a
+    pulsed 1550 nm point source."""
+    shape, eps, p = spec
+    grid = fdtd.Grid(shape, grid_spacing=50e-9, permittivity=eps[..., None], courant_number=3 ** -0.5)
+    for k in PML_KEYS(BATCH_PML):
+        grid[k] = fdtd.PML()
+    grid[p[0], p[1], p[2]] = fdtd.PointSource(period=1550e-9 / 299792458.0, pulse=True)
+    return grid
+
+
+def batch_oracle(orc, spec):
+    """The same grid through the oracle's registration API."""
+    shape, eps, p = spec
+    sim = orc.Sim(shape, 50e-9, eps.numpy()[..., None], 1.0, 3 ** -0.5)
+    for k in PML_KEYS(BATCH_PML):
+        sim.add("PML", k)
+    sim.add("PointSource", p, period=1550e-9 / 299792458.0, pulse=True)
+    return sim
+
+
+def batch_grids(fdtd, torch, count=64, seed=0):
+    """The batching configuration: `count` grids (batch_specs, batch_grid)."""
+    return [batch_grid(fdtd, spec) for spec in batch_specs(count, seed)]
 
 
 def batch_measure(steps=2000, seq_steps=200, count=64):
@@ -396,7 +491,7 @@ def main():
     ap.add_argument("--dtype", default="float64", choices=["float64", "float32", "float32c"])
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--pml", type=int, default=16)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
     ap.add_argument("--no-fp32", action="store_true", help="skip the float32 measurement")
@@ -432,10 +527,17 @@ def main():
     import fdtd_amd as fdtd
 
     n, pml = args.n, args.pml
-    main = measure(torch, fdtd, args.dtype, n, pml, args.steps, args.warmup, args.kernel_reps)
-    other = None
-    if args.dtype == "float64" and not args.no_fp32:
-        other = measure(torch, fdtd, "float32", n, pml, args.steps, args.warmup, args.kernel_reps)
+    legs = args.dtype == "float64" and not args.no_fp32
+    main = measure(torch, fdtd, args.dtype, n, pml, args.steps, args.warmup, args.kernel_reps, keep_fields=legs)
+    others = {}
+    if legs:
+        # the float32 forms on the same workload and step count, each with its
+        # error against the float64 run's final fields
+        for name in ("float32", "float32c"):
+            m = measure(torch, fdtd, name, n, pml, args.steps, args.warmup, args.kernel_reps, keep_fields=True)
+            m["error_vs_f64"] = field_errors(m.pop("fields"), main["fields"])
+            others[name] = m
+    main.pop("fields", None)
     copy_gbs = copy_bandwidth(torch)
 
     base = None
@@ -461,14 +563,23 @@ def main():
         "config": {"workload": f"{n}^3 grid, {pml}-cell PML x6, 24 dielectric spheres, pulsed point source, {storage}",
                    "grid": [n, n, n], "pml": pml, "storage": storage, "parallelism": "single GPU"},
         "step_hbm_roofline_frac": round(main["step_frac"], 4),
+        "windows_ms_per_step": main["windows_ms_per_step"],
         "copy_GBps": round(copy_gbs, 1),
         "roofline": main["roofline"],
         "cpu_baseline": base,
     }
-    if other is not None:
-        out["fp32"] = {"value": round(other["mcells"], 1), "unit": "Mcells/s", "ms_per_step": round(other["ms_step"], 4),
-                       "step_hbm_roofline_frac": round(other["step_frac"], 4), "roofline": other["roofline"],
-                       "speedup_vs_f64": round(other["mcells"] / main["mcells"], 3)}
+    for name, key, storage_ in (("float32", "fp32", "f32 fields, materials and CPML state, f32 arithmetic"),
+                                ("float32c", "fp32c", "f32 + 8 compensation bits per field value, f64 arithmetic "
+                                                      "and CPML state, 1-byte material palette")):
+        if name not in others:
+            continue
+        o = others[name]
+        out[key] = {"value": round(o["mcells"], 1), "unit": "Mcells/s", "ms_per_step": round(o["ms_step"], 4),
+                    "windows_ms_per_step": o["windows_ms_per_step"], "storage": storage_,
+                    "step_hbm_roofline_frac": round(o["step_frac"], 4), "roofline": o["roofline"],
+                    "speedup_vs_f64": round(o["mcells"] / main["mcells"], 3),
+                    "error_vs_f64": dict(o["error_vs_f64"], steps=o["steps_run"],
+                                         against="hip.float64 run of the same workload and steps")}
     if not args.no_adjoint:
         out["adjoint"] = adjoint_measure("float32")
     if not args.no_batch:

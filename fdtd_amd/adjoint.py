@@ -78,8 +78,12 @@ class IntensityGradient:
     gradient with respect to the permittivity of the design object.
 
     grid:        a HIP-backend grid with PML faces, plain or absorbing
-                 objects and sources (no periodic boundaries, no anisotropic
-                 objects, float64 or float32 storage); detectors are ignored
+                 objects and point, line or plane sources (no periodic
+                 boundaries, no anisotropic objects, no soft sources, float64
+                 or float32 storage); detectors are ignored. A plane source
+                 assigns E and H at its cells (sources.py:476-486): those
+                 values do not depend on the state, so the transposed half
+                 steps zero the adjoint fields there.
     design:      a plain ``Object`` registered in ``grid``
     box:         the objective box, a grid index tuple (x, y, z)
     steps:       T
@@ -99,9 +103,9 @@ class IntensityGradient:
             if obj is not design and _overlap(obj, design):
                 raise ValueError("another object overlaps the design region")
         for src in grid.sources:
-            if type(src).__name__ == "SoftArbitraryPointSource" and src.impedance > 0:
-                raise NotImplementedError("a resistive soft source feeds the fields back into the source: "
-                                          "its transpose is not implemented")
+            if type(src).__name__ == "SoftArbitraryPointSource":
+                raise NotImplementedError("soft sources are not supported by the gradient (a resistive one feeds "
+                                          "the fields back into the source; its transpose is not implemented)")
         if checkpoint not in ("sqrt", "record", "auto"):
             raise ValueError("checkpoint must be 'sqrt', 'record' or 'auto'")
         self.grid = grid

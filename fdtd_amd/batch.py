@@ -28,6 +28,14 @@ from . import _native
 _TUNED = {}
 
 
+def _same_key(a, b):
+    """Entry by entry: the same plan and waveform objects at the same table
+    address and length."""
+    if b is None or len(a) != len(b):
+        return False
+    return all(pa is pb and wa is wb and ka == kb for (pa, wa, ka), (pb, wb, kb) in zip(a, b))
+
+
 def _signature(grids):
     g0 = grids[0]
     return (str(g0.dtype), bool(g0.compensated),
@@ -58,12 +66,16 @@ class GridBatch:
     # ------------------------------------------------------------ plumbing
     def _ready(self, nsteps):
         plans = [g._prepare(nsteps, nsteps, nsteps) for g in self.grids]
-        # the batch holds copies of each plan's arguments, waveform table included
-        ids = tuple((id(p), id(g._wave)) for p, g in zip(plans, self.grids))
-        if self._batch is None or ids != self._plan_ids:
+        # The batch holds copies of each plan's arguments, the waveform
+        # table's address and length included. The key keeps references to
+        # the plans and tables it was built from, so that none of them can be
+        # freed (and its address reused) while the batch may still point there.
+        key = tuple((p, g._wave, None if g._wave is None else (g._wave.data_ptr(), g._wave.shape[1]))
+                    for p, g in zip(plans, self.grids))
+        if self._batch is None or not _same_key(key, self._plan_ids):
             self._batch = _native.Batch(plans)
             self._batch.set_launch(*self.launch)
-            self._plan_ids = ids
+            self._plan_ids = key
         return self._batch
 
     def _state(self):
@@ -99,8 +111,11 @@ class GridBatch:
         n = int(total_time)
         if n <= 0:
             return
-        batch = self._ready(n)
         q0 = self.time_steps_passed
+        if any(g.time_steps_passed != q0 for g in self.grids):
+            raise ValueError("batched grids must be at the same time step "
+                             f"(found {sorted({g.time_steps_passed for g in self.grids})})")
+        batch = self._ready(n)
         stream = torch.cuda.current_stream().cuda_stream
         if graph and n >= self.graph_steps:
             batch.run_graph(q0, n, self.graph_steps, stream)

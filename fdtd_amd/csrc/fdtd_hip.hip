@@ -16,6 +16,7 @@
 
 using namespace fdtd_dev;
 
+
 namespace {
 
 thread_local std::string g_err;
@@ -54,6 +55,7 @@ struct SrcHost {
   void* vin = nullptr;
   void* vout = nullptr;
   int64_t cap = 0, count = 0;
+  const int64_t* d_cells = nullptr;  // the cells in the plan's device table (set by finalize)
 };
 struct CurHost {
   std::vector<int32_t> pts;  // px[n], py[n], pz[n]
@@ -432,7 +434,22 @@ int adj_check_plan(const FdtdPlan* p) {
     if (b.periodic) return fail(-1, "adjoint does not support periodic boundaries");
   for (const ObjHost& o : p->objs)
     if (o.kind == FDTD_OBJ_ANISOTROPIC) return fail(-1, "adjoint does not support anisotropic objects");
+  for (const SrcHost& s : p->srcs)
+    if (s.kind == FDTD_SRC_SOFT) return fail(-1, "adjoint does not support soft sources");
   return 0;
+}
+
+// The hard (plane) sources overwrite E and H at their cells after each half
+// step (sources.py:476-486), so those values do not depend on the state
+// before: the transposed half step starts by zeroing the adjoint field there.
+template <typename T>
+void adj_mask_hard_sources(const FdtdPlan* p, T* lam, bool is_H, hipStream_t st) {
+  for (const SrcHost& s : p->srcs) {
+    if (s.kind != FDTD_SRC_PLANE) continue;
+    const int n = (int)s.cells.size();
+    const int64_t base = (int64_t)(is_H ? s.compH : s.compE) * p->d.cs;
+    hipLaunchKernelGGL(k_adj_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lam + base, s.d_cells, n);
+  }
 }
 
 bool box_ok(const FdtdPlan* p, const int32_t* b) {
@@ -509,10 +526,12 @@ int adjoint_steps(FdtdAdjoint* a, int64_t nsteps, const T* curl, const T* plane,
   for (int64_t s = nsteps - 1; s >= 0; --s) {
     args.plane = plane + s * 3 * np;
     args.side = adj_sides<T>(a, true, a->parH);
+    adj_mask_hard_sources<T>(p, args.lamH, true, st);
     hipLaunchKernelGGL(k_adj_H<T>, grid, block, 0, st, args);
     a->parH ^= 1;
     args.curl = curl + s * 3 * nd;
     args.side = adj_sides<T>(a, false, a->parE);
+    adj_mask_hard_sources<T>(p, args.lamE, false, st);
     hipLaunchKernelGGL(k_adj_E<T>, grid, block, 0, st, args);
     a->parE ^= 1;
   }
@@ -558,6 +577,42 @@ void batch_free_graph(FdtdBatch* b) {
   b->gsteps = 0;
 }
 
+// Per-grid kernel arguments of a batch for a launch shape: the E and H main
+// arguments of every plan and first[s], the first work item of grid s
+// (first[n] = the number of work items).
+template <class FS, class MS>
+void batch_tables(const std::vector<FdtdPlan*>& plans, int block_threads, int cx, std::vector<MainArgs<FS, MS>>& sE,
+                  std::vector<MainArgs<FS, MS>>& sH, std::vector<int64_t>& first, bool& objs) {
+  using MA = MainArgs<FS, MS>;
+  const size_t n = plans.size();
+  sE.assign(n, MA{});
+  sH.assign(n, MA{});
+  first.assign(n + 1, 0);
+  objs = false;
+  for (size_t i = 0; i < n; ++i) {
+    FdtdPlan* p = plans[i];
+    MA aE = main_args<FS, MS>(p, false), aH = main_args<FS, MS>(p, true);
+    const int bzs = lanes_shift(p->d.nz);
+    for (MA* a : {&aE, &aH}) {
+      a->cx = std::max(1, std::min(cx, p->d.nx));
+      a->by = std::max(1, block_threads >> bzs);
+    }
+    if (p->fold_points) {
+      aE.npts = (int32_t)p->srcs.size();
+      for (int s = 0; s < aE.npts; ++s) aE.pt_cell[s] = p->srcs[s].cells[0];
+      aE.wave = p->wave;
+      aE.wave_cap = p->wave_cap;
+    }
+    objs = objs || aE.nobjs > 0;
+    sE[i] = aE;
+    sH[i] = aH;
+    const int bz = 1 << bzs;
+    const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + aE.by - 1) / aE.by) *
+                          ((p->d.nx + aE.cx - 1) / aE.cx);
+    first[i + 1] = first[i] + tiles;
+  }
+}
+
 // (Re)build the per-grid argument tables for the current launch shape.
 int batch_upload(FdtdBatch* b) {
   FdtdPlan* p0 = b->plans[0];
@@ -569,33 +624,11 @@ int batch_upload(FdtdBatch* b) {
     b->dev = nullptr;
   }
   const size_t n = b->plans.size();
-  std::vector<int64_t> first(n + 1, 0);
-  b->objs = false;
   return with_policies(p0, [&]<class FS, class MS, typename O>() -> int {
     using MA = MainArgs<FS, MS>;
-    std::vector<MA> sE(n), sH(n);
-    for (size_t i = 0; i < n; ++i) {
-      FdtdPlan* p = b->plans[i];
-      MA aE = main_args<FS, MS>(p, false), aH = main_args<FS, MS>(p, true);
-      const int bzs = lanes_shift(p->d.nz);
-      for (MA* a : {&aE, &aH}) {
-        a->cx = std::max(1, std::min(b->cx, p->d.nx));
-        a->by = std::max(1, b->block_threads >> bzs);
-      }
-      if (p->fold_points) {
-        aE.npts = (int32_t)p->srcs.size();
-        for (int s = 0; s < aE.npts; ++s) aE.pt_cell[s] = p->srcs[s].cells[0];
-        aE.wave = p->wave;
-        aE.wave_cap = p->wave_cap;
-      }
-      b->objs = b->objs || aE.nobjs > 0;
-      sE[i] = aE;
-      sH[i] = aH;
-      const int bz = 1 << bzs;
-      const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + aE.by - 1) / aE.by) *
-                            ((p->d.nx + aE.cx - 1) / aE.cx);
-      first[i + 1] = first[i] + tiles;
-    }
+    std::vector<MA> sE, sH;
+    std::vector<int64_t> first;
+    batch_tables<FS, MS>(b->plans, b->block_threads, b->cx, sE, sH, first, b->objs);
     b->total = first[n];
     if (b->total >= (int64_t)1 << 31) return fail(-1, "batch has too many work items");
     std::vector<int32_t> item_sim(b->total);
@@ -626,16 +659,27 @@ int batch_groups(const FdtdBatch* b) {
   return (int)b->total;  // one workgroup per work item
 }
 
-// nsteps batched steps on stream st (the time index already set on the device)
-int batch_launch(FdtdBatch* b, int64_t nsteps, hipStream_t st) {
-  const dim3 grid((unsigned)batch_groups(b)), block((unsigned)b->block_threads);
-  return with_policies(b->plans[0], [&]<class FS, class MS, typename O>() -> int {
-    BatchArgs<FS, MS> aE{reinterpret_cast<const MainArgs<FS, MS>*>(b->sims_E), b->first, b->item_sim, b->total,
-                         b->q};
+// Device tables of a batched launch (see BatchArgs).
+struct BatchTables {
+  const void* sims_E;
+  const void* sims_H;
+  const int64_t* first;
+  const int32_t* item_sim;
+  int64_t total;
+  int64_t* q;
+};
+
+// nsteps batched steps on stream st (the time index already set on the
+// device), `groups` workgroups of `block_threads`
+int batch_launch_tables(const FdtdPlan* p0, const BatchTables& t, bool objs, int groups, int block_threads,
+                        int64_t nsteps, hipStream_t st) {
+  const dim3 grid((unsigned)groups), block((unsigned)block_threads);
+  return with_policies(p0, [&]<class FS, class MS, typename O>() -> int {
+    BatchArgs<FS, MS> aE{reinterpret_cast<const MainArgs<FS, MS>*>(t.sims_E), t.first, t.item_sim, t.total, t.q};
     BatchArgs<FS, MS> aH = aE;
-    aH.sims = reinterpret_cast<const MainArgs<FS, MS>*>(b->sims_H);
+    aH.sims = reinterpret_cast<const MainArgs<FS, MS>*>(t.sims_H);
     for (int64_t s = 0; s < nsteps; ++s) {
-      if (b->objs)
+      if (objs)
         hipLaunchKernelGGL((k_batch_E<FS, MS, O, true, true>), grid, block, 0, st, aE);
       else
         hipLaunchKernelGGL((k_batch_E<FS, MS, O, false, true>), grid, block, 0, st, aE);
@@ -646,14 +690,19 @@ int batch_launch(FdtdBatch* b, int64_t nsteps, hipStream_t st) {
   });
 }
 
-int batch_room(FdtdBatch* b, int64_t q0, int64_t nsteps) {
-  for (FdtdPlan* p : b->plans)
+int batch_launch(FdtdBatch* b, int64_t nsteps, hipStream_t st) {
+  const BatchTables t{b->sims_E, b->sims_H, b->first, b->item_sim, b->total, b->q};
+  return batch_launch_tables(b->plans[0], t, b->objs, batch_groups(b), b->block_threads, nsteps, st);
+}
+
+int batch_room(const std::vector<FdtdPlan*>& plans, int64_t q0, int64_t nsteps) {
+  for (FdtdPlan* p : plans)
     if (int rc = check_step_room(p, q0, nsteps, false, false)) return rc;
   return 0;
 }
 
-void batch_advance(FdtdBatch* b, int64_t nsteps) {
-  for (FdtdPlan* p : b->plans) {
+void batch_advance(const std::vector<FdtdPlan*>& plans, int64_t nsteps) {
+  for (FdtdPlan* p : plans) {
     p->nE += nsteps;
     p->nH += nsteps;
   }
@@ -1025,6 +1074,7 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     sd.compE = s.compE;
     sd.compH = s.compH;
     sd.cells = reinterpret_cast<const int64_t*>(base + o_scell[i]);
+    p->srcs[i].d_cells = sd.cells;
     sd.profile = s.profile.empty() ? nullptr : reinterpret_cast<const double*>(base + o_sprof[i]);
     sd.amplitude = s.amplitude;
     sd.spacing = s.spacing;
@@ -1253,19 +1303,19 @@ int fdtd_batch_work_items(const FdtdBatch* b, int64_t* items) {
 int fdtd_batch_run(FdtdBatch* b, int64_t q0, int64_t nsteps, void* stream) {
   if (!b) return fail(-1, "null batch");
   if (nsteps < 0) return fail(-1, "negative step count");
-  if (int rc = batch_room(b, q0, nsteps)) return rc;
+  if (int rc = batch_room(b->plans, q0, nsteps)) return rc;
   if (int rc = select_device(b->plans[0])) return rc;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_set_time, dim3(1), dim3(64), 0, st, b->q, q0);
   if (int rc = batch_launch(b, nsteps, st)) return rc;
-  batch_advance(b, nsteps);
+  batch_advance(b->plans, nsteps);
   return 0;
 }
 
 int fdtd_batch_run_graph(FdtdBatch* b, int64_t q0, int64_t nsteps, int64_t steps_per_graph, void* stream) {
   if (!b) return fail(-1, "null batch");
   if (nsteps < 0 || steps_per_graph < 1) return fail(-1, "bad step counts");
-  if (int rc = batch_room(b, q0, nsteps)) return rc;
+  if (int rc = batch_room(b->plans, q0, nsteps)) return rc;
   if (int rc = select_device(b->plans[0])) return rc;
   hipStream_t caller = reinterpret_cast<hipStream_t>(stream);
   if (!b->gstream) {
@@ -1302,7 +1352,7 @@ int fdtd_batch_run_graph(FdtdBatch* b, int64_t q0, int64_t nsteps, int64_t steps
     if (int rc = batch_launch(b, rest, b->gstream)) return rc;
   FDTD_HIP(hipEventRecord(b->ev_out, b->gstream));
   FDTD_HIP(hipStreamWaitEvent(caller, b->ev_out, 0));
-  batch_advance(b, nsteps);
+  batch_advance(b->plans, nsteps);
   return 0;
 }
 

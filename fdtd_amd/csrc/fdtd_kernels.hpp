@@ -1106,6 +1106,13 @@ __device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int i, int j, i
   }
 }
 
+// Zeroes lam at n cells (one component; lam already offset to it).
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_zero(T* lam, const int64_t* cells, int n) {
+  const int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p < n) lam[cells[p]] = T(0);
+}
+
 // Transpose of the H half step (grid.py:301-325, boundaries.py:421-431 and
 // 461-487). Forward, per face normal to ax and psi component q (H component
 // C_q, E component s_q = 3-ax-C_q differenced along ax):

@@ -155,9 +155,10 @@ class Grid:
             return
         if not torch.is_tensor(value):
             value = torch.as_tensor(np.asarray(value))
-        if self.compensated:
+        if self.compensated and (dst is self._E or dst is self._H):
             # whole-field assignment keeps 32 significant bits: the float part
-            # and the next 8 mantissa bits (rounded at bit 21 of the double)
+            # and the next 8 mantissa bits (rounded at bit 21 of the double);
+            # the material arrays are plain float64 and take the copy below
             v = value.to(device=dst.device, dtype=torch.float64).expand_as(view)
             bits = v.contiguous().view(torch.int64) + (1 << 20)
             q = (bits >> 21) & 0xFF  # (Nx, Ny, Nz, 3)

@@ -61,6 +61,12 @@ TINY_ABSORBER = dict(TINY, name="tiny_absorber",
                      absorber=((slice(14, 17), slice(4, 14), slice(3, 11)), 2.0, 1e4))
 
 
+# TINY plus a hard PlaneSource (x = 10) that cuts through the design region:
+# its cells' E (polarisation y) and H components are assigned every half step
+TINY_PLANE = dict(TINY, name="tiny_plane",
+                  plane_source=((10, slice(4, 14), slice(2, 12)), dict(period=12, amplitude=0.3, polarization="y")))
+
+
 def design_shape(sc):
     return tuple(s.stop - s.start for s in sc["design"])
 
@@ -88,6 +94,9 @@ def build(mod, sc, permittivity):
     if sc.get("absorber"):
         key, eps, sigma = sc["absorber"]
         grid[key] = mod.AbsorbingObject(permittivity=eps, conductivity=sigma, name="absorber")
+    if sc.get("plane_source"):
+        key, kw = sc["plane_source"]
+        grid[key] = mod.PlaneSource(name="plane_source", **kw)
     return grid, design
 
 
@@ -113,6 +122,9 @@ def build_oracle(oracle_mod, sc, permittivity):
     if sc.get("absorber"):
         key, eps, sigma = sc["absorber"]
         sim.add("AbsorbingObject", key, permittivity=eps, conductivity=sigma)
+    if sc.get("plane_source"):
+        key, kw = sc["plane_source"]
+        sim.add("PlaneSource", key, **kw)
     return sim
 
 

@@ -14,6 +14,7 @@ import adjoint_scenarios as A
 import fdtd_amd as fdtd
 from fdtd_amd.adjoint import IntensityGradient, _box_of
 from fdtd_amd.backend import host_only
+from fdtd_amd.sources import SoftArbitraryPointSource
 from oracle import fdtd_oracle as orc
 
 
@@ -105,6 +106,11 @@ def test_gradient_api_validation(host):
         g2[0:2, 0:2, 0:2] = fdtd.AnisotropicObject(2.0, name="aniso")
         with pytest.raises(NotImplementedError):
             IntensityGradient(g2, d2, sc["plane"], 10)
+        # soft sources have no transposed step
+        g4, d4 = A.build(fdtd, sc, torch.from_numpy(A.design_permittivity(sc)))
+        g4[3, 3, 3] = SoftArbitraryPointSource(waveform_array=np.zeros(8), impedance=0.0, name="soft")
+        with pytest.raises(NotImplementedError):
+            IntensityGradient(g4, d4, sc["plane"], 10)
         fdtd.set_backend("hip.float32c")
         g3, d3 = A.build(fdtd, sc, torch.from_numpy(A.design_permittivity(sc)))
         with pytest.raises(NotImplementedError):

@@ -112,6 +112,25 @@ def test_small_case_with_absorber_matches_central_differences():
     assert e <= 1e-6
 
 
+def test_small_case_with_plane_source_matches_central_differences():
+    """A hard PlaneSource through the design region: its E and H assignments
+    (sources.py:476-486) cut the dependence on the state at its cells, which
+    the transposed half steps reproduce by zeroing the adjoint fields there;
+    without that mask the gradient at the plane's voxels is wrong."""
+    sc = A.TINY_PLANE
+    prob, perm = _problem(sc)
+    J, g = prob.gradient(torch.from_numpy(perm))
+    g = g.cpu().numpy()
+    voxels = [tuple(int(i) for i in v) for v in np.ndindex(*perm.shape)]
+    fd = _central_differences(prob, perm, voxels)
+    ga = np.array([g[v] for v in voxels])
+    e = rel(ga, fd)
+    on_plane = np.array([v[0] + sc["design"][0].start == 10 for v in voxels])
+    e_plane = rel(ga[on_plane], fd[on_plane])
+    _say(f"[gpu] adjoint tiny_plane: {len(voxels)} voxels, rel L2 {e:.2e}, on the plane {e_plane:.2e}")
+    assert e <= 1e-6 and e_plane <= 1e-6
+
+
 def test_c5_float64_gradient_matches_central_differences():
     sc = A.C5
     prob, perm = _problem(sc)

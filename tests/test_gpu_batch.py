@@ -116,3 +116,83 @@ def test_batch_rejects_ineligible_grids():
     g3[6, 6, 6] = fdtd.PointSource(period=10.0)
     with pytest.raises(RuntimeError):
         GridBatch([g1, g3]).run(2)
+
+
+# ------------------------------------------------------------------ C4
+# The batching configuration itself (bench.batch_specs / bench.batch_grid):
+# 64 grids with edges drawn from 64-192 cells (uneven z rows), smoothed-noise
+# permittivity in [1, 12], 10-cell PML on all faces, a pulsed point source.
+
+def _zero_state(batch):
+    for t in batch._state():
+        t.zero_()
+    for g in batch.grids:
+        g.time_steps_passed = 0
+
+
+def test_c4_batch_matches_grids_stepped_alone_and_oracle():
+    """float32, 50 steps: the 64-grid batch (graph replays of 10 steps with
+    the default launch, direct launches of a persistent launch shape) bit-
+    identical to every grid stepped alone; the three smallest grids' float32
+    error against the oracle stated and bounded."""
+    import bench
+    from oracle import fdtd_oracle as orc
+
+    fdtd.set_backend("hip.float32")
+    steps = 50
+    specs = bench.batch_specs(64)
+    grids = [bench.batch_grid(fdtd, s) for s in specs]
+    _say(f"[gpu] c4 batch: {len(grids)} grids, {sum(g.Nx * g.Ny * g.Nz for g in grids)} cells, "
+         f"nz from {min(g.Nz for g in grids)} to {max(g.Nz for g in grids)}")
+    for g in grids:
+        g.run(steps, progress_bar=False)
+    want = [(g._E.clone(), g._H.clone()) for g in grids]
+    batch = GridBatch(grids, graph_steps=10)
+    for graph, shape in ((True, (256, 2, 0)), (False, (256, 1, 2048))):
+        _zero_state(batch)
+        batch.set_launch(*shape)
+        _say(f"[gpu] c4 batch float32 graph={graph} launch={shape}")
+        batch.run(steps, graph=graph)
+        torch.cuda.synchronize()
+        for i, (g, (ew, hw)) in enumerate(zip(grids, want)):
+            assert torch.equal(g._E, ew) and torch.equal(g._H, hw), f"grid {i} {g.shape} differs ({shape})"
+        assert all(g.time_steps_passed == steps for g in grids)
+    smallest = sorted(range(len(specs)), key=lambda i: np.prod(specs[i][0]))[:3]
+    for i in smallest:
+        sim = bench.batch_oracle(orc, specs[i])
+        for _ in range(steps):
+            sim.step()
+        ref = np.concatenate([sim.E.reshape(-1), sim.H.reshape(-1)])
+        got = np.concatenate([grids[i].E.double().cpu().numpy().reshape(-1),
+                              grids[i].H.double().cpu().numpy().reshape(-1)])
+        e = np.linalg.norm(got - ref) / np.linalg.norm(ref)
+        m = np.abs(got - ref).max() / np.abs(ref).max()
+        _say(f"[gpu] c4 grid {i} {specs[i][0]} float32 batch vs oracle, {steps} steps: rel L2 {e:.2e}, "
+             f"max-norm {m:.2e}")
+        assert e <= 1e-4 and m <= 1e-4
+    fdtd.set_backend("hip.float64")
+
+
+def test_c4_batch_float64_smallest_grids_bit_exact_against_oracle():
+    """float64, 40 steps of the whole 64-grid batch (graph replays): the three
+    smallest grids bit for bit against the oracle."""
+    import bench
+    from oracle import fdtd_oracle as orc
+
+    fdtd.set_backend("hip.float64")
+    steps = 40
+    specs = bench.batch_specs(64)
+    grids = [bench.batch_grid(fdtd, s) for s in specs]
+    GridBatch(grids, graph_steps=10).run(steps)
+    torch.cuda.synchronize()
+    smallest = sorted(range(len(specs)), key=lambda i: np.prod(specs[i][0]))[:3]
+    for i in smallest:
+        sim = bench.batch_oracle(orc, specs[i])
+        for _ in range(steps):
+            sim.step()
+        eq_E = np.array_equal(grids[i].E.cpu().numpy(), sim.E)
+        eq_H = np.array_equal(grids[i].H.cpu().numpy(), sim.H)
+        _say(f"[gpu] c4 grid {i} {specs[i][0]} float64 batch vs oracle, {steps} steps: bit-exact E {eq_E}, H {eq_H}, "
+             f"max |E| {np.abs(sim.E).max():.3e}")
+        assert np.abs(sim.E).max() > 0
+        assert eq_E and eq_H

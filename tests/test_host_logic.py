@@ -350,3 +350,28 @@ def test_compensated_records_and_host_writes():
         assert grid._E_lo.data_ptr() == grid._E_rec.data_ptr() + 12
     finally:
         fdtd.set_backend("hip.float64")
+
+
+def test_compensated_material_assignment_keeps_fields_and_precision():
+    """On hip.float32c, assigning a grid material writes the float64 material
+    array exactly and leaves the fields' compensation words alone (only E and
+    H are stored as float + 8 bits)."""
+    fdtd.set_backend("hip.float32c")
+    try:
+        grid = fdtd.Grid((4, 5, 6), grid_spacing=50e-9, courant_number=0.5)
+        grid.H = torch.full((4, 5, 6, 3), 1.0 + 2.0 ** -30, dtype=torch.float64)
+        grid.E = torch.full((4, 5, 6, 3), -3.0 - 2.0 ** -28, dtype=torch.float64)
+        h_before = grid.full_precision("H").clone()
+        e_before = grid.full_precision("E").clone()
+        lo_before = (grid._E_lo.clone(), grid._H_lo.clone())
+        assert float(h_before[0, 0, 0, 0]) == 1.0 + 2.0 ** -30
+        grid.inverse_permittivity = 1.0 / 3.0
+        grid.inverse_permeability = torch.full((4, 5, 6, 3), 0.7, dtype=torch.float64)
+        assert grid.inverse_permittivity.dtype == torch.float64
+        assert torch.all(grid.inverse_permittivity == 1.0 / 3.0)
+        assert torch.all(grid.inverse_permeability == 0.7)
+        assert torch.equal(grid._E_lo, lo_before[0]) and torch.equal(grid._H_lo, lo_before[1])
+        assert torch.equal(grid.full_precision("H"), h_before)
+        assert torch.equal(grid.full_precision("E"), e_before)
+    finally:
+        fdtd.set_backend("hip.float64")
