@@ -112,6 +112,7 @@ struct FdtdPlan {
   bool fold_points = false;  // point sources added inside the E kernel
   bool nontemporal = true;   // read-once streams marked non-temporal
   bool cyclic = false;       // x-chunks dealt cyclically to the XCDs (large grids)
+  bool flat = false;         // flat tiles: block_threads consecutive cells of the (y, z) plane
   // FDTD_F32C: 8-bit compensation terms of E and H
   void* E_lo = nullptr;
   void* H_lo = nullptr;
@@ -266,17 +267,23 @@ MainArgs<FS, MS> main_args(const FdtdPlan* p, bool is_H) {
   a.cx = p->cx;
   a.by = std::max(1, p->block_threads >> bzs);
   a.cyclic = p->cyclic;
+  a.flat = p->flat ? p->block_threads : 0;
   a.npts = 0;
   return a;
 }
 
+// threads per workgroup of main-kernel arguments a
+template <class FS, class MS>
+unsigned block_of(const MainArgs<FS, MS>& a) {
+  return a.flat ? (unsigned)a.flat : (unsigned)((1 << a.bz_shift) * a.by);
+}
+
 template <class FS, class MS, typename O>
 void launch_main(const FdtdPlan* p, const MainArgs<FS, MS>& a, bool is_H, hipStream_t st) {
-  const int bz = 1 << a.bz_shift, by = a.by;
-  const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
+  const unsigned tiles = plane_tiles(p->d, a.bz_shift, a.by, a.flat);
   const unsigned nchunks = (unsigned)((p->d.nx + a.cx - 1) / a.cx);
   // x-chunks dealt cyclically to the 8 XCDs (xcd_chunk_tile): whole rounds of 8
-  const dim3 grid(tiles * (a.cyclic ? (nchunks + 7) / 8 * 8 : nchunks)), block(bz * by);
+  const dim3 grid(tiles * (a.cyclic ? (nchunks + 7) / 8 * 8 : nchunks)), block(block_of(a));
   if (a.rec) {  // recording forward pass of a gradient (plain storage)
     if (a.nobjs)
       hipLaunchKernelGGL((k_main_E<FS, MS, O, true, true, true>), grid, block, 0, st, a);
@@ -568,6 +575,7 @@ int batch_check_plan(const FdtdPlan* p, const FdtdPlan* p0) {
   if (!p->stages.empty()) return fail(-1, "batched grids cannot have periodic boundaries");
   if (!p->dets.empty()) return fail(-1, "batched grids cannot have detectors");
   if (!p->srcs.empty() && !p->fold_points) return fail(-1, "batched grids may have at most 4 point sources only");
+  if (p->device != p0->device) return fail(-1, "batched grids must be on one device");
   return 0;
 }
 
@@ -596,6 +604,7 @@ void batch_tables(const std::vector<FdtdPlan*>& plans, int block_threads, int cx
     for (MA* a : {&aE, &aH}) {
       a->cx = std::max(1, std::min(cx, p->d.nx));
       a->by = std::max(1, block_threads >> bzs);
+      a->flat = p->flat ? block_threads : 0;
     }
     if (p->fold_points) {
       aE.npts = (int32_t)p->srcs.size();
@@ -606,11 +615,45 @@ void batch_tables(const std::vector<FdtdPlan*>& plans, int block_threads, int cx
     objs = objs || aE.nobjs > 0;
     sE[i] = aE;
     sH[i] = aH;
-    const int bz = 1 << bzs;
-    const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + aE.by - 1) / aE.by) *
-                          ((p->d.nx + aE.cx - 1) / aE.cx);
+    const int64_t tiles = (int64_t)plane_tiles(p->d, aE.bz_shift, aE.by, aE.flat) * ((p->d.nx + aE.cx - 1) / aE.cx);
     first[i + 1] = first[i] + tiles;
   }
+}
+
+// Host image of a batch's device tables for a launch shape: the E and H
+// argument tables, first[], the time index slot and item_sim[], at the byte
+// offsets below (each 256-byte aligned).
+struct BatchImage {
+  std::vector<unsigned char> bytes;
+  size_t oE = 0, oH = 0, oF = 0, oQ = 0, oI = 0;
+  int64_t total = 0;  // work items
+  bool objs = false;
+};
+
+int batch_image(const std::vector<FdtdPlan*>& plans, int block_threads, int cx, BatchImage& img) {
+  const size_t n = plans.size();
+  return with_policies(plans[0], [&]<class FS, class MS, typename O>() -> int {
+    using MA = MainArgs<FS, MS>;
+    std::vector<MA> sE, sH;
+    std::vector<int64_t> first;
+    batch_tables<FS, MS>(plans, block_threads, cx, sE, sH, first, img.objs);
+    img.total = first[n];
+    if (img.total >= (int64_t)1 << 31) return fail(-1, "batch has too many work items");
+    img.oE = 0;
+    img.oH = align_up(n * sizeof(MA));
+    img.oF = img.oH + align_up(n * sizeof(MA));
+    img.oQ = img.oF + align_up((n + 1) * sizeof(int64_t));
+    img.oI = img.oQ + 256;
+    img.bytes.assign(img.oI + align_up(img.total * sizeof(int32_t)), 0);
+    unsigned char* h = img.bytes.data();
+    std::memcpy(h + img.oE, sE.data(), n * sizeof(MA));
+    std::memcpy(h + img.oH, sH.data(), n * sizeof(MA));
+    std::memcpy(h + img.oF, first.data(), (n + 1) * sizeof(int64_t));
+    int32_t* item_sim = reinterpret_cast<int32_t*>(h + img.oI);
+    for (size_t i = 0; i < n; ++i)
+      for (int64_t t = first[i]; t < first[i + 1]; ++t) item_sim[t] = (int32_t)i;
+    return 0;
+  });
 }
 
 // (Re)build the per-grid argument tables for the current launch shape.
@@ -623,35 +666,19 @@ int batch_upload(FdtdBatch* b) {
     (void)hipFree(b->dev);
     b->dev = nullptr;
   }
-  const size_t n = b->plans.size();
-  return with_policies(p0, [&]<class FS, class MS, typename O>() -> int {
-    using MA = MainArgs<FS, MS>;
-    std::vector<MA> sE, sH;
-    std::vector<int64_t> first;
-    batch_tables<FS, MS>(b->plans, b->block_threads, b->cx, sE, sH, first, b->objs);
-    b->total = first[n];
-    if (b->total >= (int64_t)1 << 31) return fail(-1, "batch has too many work items");
-    std::vector<int32_t> item_sim(b->total);
-    for (size_t i = 0; i < n; ++i)
-      for (int64_t t = first[i]; t < first[i + 1]; ++t) item_sim[t] = (int32_t)i;
-    const size_t oE = 0, oH = align_up(n * sizeof(MA)), oF = oH + align_up(n * sizeof(MA)),
-                 oQ = oF + align_up((n + 1) * sizeof(int64_t)), oI = oQ + 256,
-                 total = oI + align_up(b->total * sizeof(int32_t));
-    std::vector<unsigned char> host(total, 0);
-    std::memcpy(host.data() + oE, sE.data(), n * sizeof(MA));
-    std::memcpy(host.data() + oH, sH.data(), n * sizeof(MA));
-    std::memcpy(host.data() + oF, first.data(), (n + 1) * sizeof(int64_t));
-    std::memcpy(host.data() + oI, item_sim.data(), b->total * sizeof(int32_t));
-    FDTD_HIP(hipMalloc(&b->dev, total));
-    FDTD_HIP(hipMemcpy(b->dev, host.data(), total, hipMemcpyHostToDevice));
-    char* base = reinterpret_cast<char*>(b->dev);
-    b->sims_E = base + oE;
-    b->sims_H = base + oH;
-    b->first = reinterpret_cast<int64_t*>(base + oF);
-    b->q = reinterpret_cast<int64_t*>(base + oQ);
-    b->item_sim = reinterpret_cast<int32_t*>(base + oI);
-    return 0;
-  });
+  BatchImage img;
+  if (int rc = batch_image(b->plans, b->block_threads, b->cx, img)) return rc;
+  b->total = img.total;
+  b->objs = img.objs;
+  FDTD_HIP(hipMalloc(&b->dev, img.bytes.size()));
+  FDTD_HIP(hipMemcpy(b->dev, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice));
+  char* base = reinterpret_cast<char*>(b->dev);
+  b->sims_E = base + img.oE;
+  b->sims_H = base + img.oH;
+  b->first = reinterpret_cast<int64_t*>(base + img.oF);
+  b->q = reinterpret_cast<int64_t*>(base + img.oQ);
+  b->item_sim = reinterpret_cast<int32_t*>(base + img.oI);
+  return 0;
 }
 
 int batch_groups(const FdtdBatch* b) {
@@ -693,6 +720,13 @@ int batch_launch_tables(const FdtdPlan* p0, const BatchTables& t, bool objs, int
 int batch_launch(FdtdBatch* b, int64_t nsteps, hipStream_t st) {
   const BatchTables t{b->sims_E, b->sims_H, b->first, b->item_sim, b->total, b->q};
   return batch_launch_tables(b->plans[0], t, b->objs, batch_groups(b), b->block_threads, nsteps, st);
+}
+
+// Pointers to the tables of an image placed at device address base.
+BatchTables image_tables(const BatchImage& img, void* base) {
+  char* c = reinterpret_cast<char*>(base);
+  return BatchTables{c + img.oE, c + img.oH, reinterpret_cast<const int64_t*>(c + img.oF),
+                     reinterpret_cast<const int32_t*>(c + img.oI), img.total, reinterpret_cast<int64_t*>(c + img.oQ)};
 }
 
 int batch_room(const std::vector<FdtdPlan*>& plans, int64_t q0, int64_t nsteps) {
@@ -967,8 +1001,14 @@ int fdtd_plan_finalize(FdtdPlan* p) {
       if (bt == 256 || bt == 512 || bt == 1024) p->block_threads = bt;
     }
     const int bzs = lanes_shift(p->d.nz);
-    const int bz = 1 << bzs, by = std::max(1, p->block_threads >> bzs);
-    const int64_t tiles = (int64_t)((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by);
+    const int by = std::max(1, p->block_threads >> bzs);
+    // flat tiles where z-rows of whole wavefronts would leave lanes idle
+    // (nz not a multiple of 64); planes of 2^31 cells and more keep row tiles
+    const int64_t plane = (int64_t)p->d.ny * p->d.nz;
+    p->flat = p->d.nz % 64 != 0;
+    if (const char* env = getenv("FDTD_HIP_FLAT")) p->flat = atoi(env) != 0;
+    p->flat = p->flat && plane < ((int64_t)1 << 31);
+    const int64_t tiles = plane_tiles(p->d, bzs, by, p->flat ? p->block_threads : 0);
     // measured on MI355X: plain storage is fastest at 2 planes per workgroup
     // from 256^3 up to 512x512x384 (14 % faster there than at 12) and at 1 on
     // grids of ~1M cells; the compensated kernels at 64 x-chunks (4 planes at

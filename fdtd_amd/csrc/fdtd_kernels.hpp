@@ -513,6 +513,8 @@ struct MainArgs {
   int32_t cx;        // planes per workgroup along x
   int32_t by;        // block extent along y (threads per block = bz * by)
   int32_t cyclic;    // single-grid launch: x-chunks dealt cyclically to the XCDs
+  int32_t flat;      // 0: tiles of bz z-cells by `by` z-rows; else tiles of `flat`
+                     // (= threads per block) consecutive cells of the (y, z) plane
   // point sources folded into the E kernel (no periodic stage, no detector):
   // their cells, in registration order, and the waveform table
   int32_t npts;
@@ -526,25 +528,61 @@ struct MainArgs {
   Box rb;
 };
 
-// Main E kernel. A workgroup owns a (y, z) tile of bz*by columns (z fastest,
-// one row of bz cells per wavefront when bz = 64) and marches along x over a
-// chunk of `cx` planes: the x-neighbours come from registers of the previous
-// plane, the y- and z-neighbours from cache (the rows were just streamed in
-// by this or the neighbouring wavefront). Every load of a plane is issued
-// unconditionally and at once, so a plane costs one memory round trip:
-// threads past the grid's edge read the nearest cell and store nothing, and
-// at the low edges the neighbour offset is 0 (the difference is dropped).
+// Tiles of one x-chunk: row tiles (bz z-cells by `by` z-rows) or flat tiles
+// (`flat` consecutive cells of the (y, z) plane, wrapping from the end of one
+// z-row to the start of the next, so that only the plane's last tile has
+// idle lanes whatever nz is).
+__host__ __device__ __forceinline__ unsigned plane_tiles(const Dims& d, int bz_shift, int by, int flat) {
+  if (flat) return (unsigned)(((int64_t)d.ny * d.nz + flat - 1) / flat);
+  const int bz = 1 << bz_shift;
+  return (unsigned)(((d.nz + bz - 1) / bz) * ((d.ny + by - 1) / by));
+}
+
+// This thread's column (j, k) in tile `tile`, the tile's x range [i0, i1),
+// and whether the column is inside the grid: threads past the edge take the
+// nearest cell (so that their loads stay in bounds) and store nothing.
+template <class A>
+__device__ __forceinline__ void tile_column(const A& a, unsigned tile, int& j, int& k, int& i0, int& i1, bool& act) {
+  const Dims& d = a.d;
+  unsigned tx;
+  if (a.flat) {
+    const int plane = d.ny * d.nz;  // the host uses flat tiles for planes below 2^31 cells
+    const unsigned per = (unsigned)((plane + a.flat - 1) / a.flat);
+    const int p0 = (int)(tile % per) * a.flat + (int)threadIdx.x;
+    tx = tile / per;
+    act = p0 < plane;
+    const int p = min(p0, plane - 1);
+    j = p / d.nz;
+    k = p - j * d.nz;
+  } else {
+    const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
+    const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
+    const int tz = tile % ntz, ty = (tile / ntz) % nty;
+    tx = tile / (ntz * nty);
+    const int k0 = tz * bz + (threadIdx.x & (bz - 1)), j0 = ty * by + (threadIdx.x >> bzs);
+    act = k0 < d.nz && j0 < d.ny;
+    k = min(k0, d.nz - 1);
+    j = min(j0, d.ny - 1);
+  }
+  i0 = (int)tx * a.cx;
+  i1 = min(i0 + a.cx, d.nx);
+}
+
+// Main E kernel. A workgroup owns a (y, z) tile (tile_column) and marches
+// along x over a chunk of `cx` planes: the x-neighbours come from registers
+// of the previous plane, the y- and z-neighbours from cache (the rows were
+// just streamed in by this or the neighbouring wavefront). Every load of a
+// plane is issued unconditionally and at once, so a plane costs one memory
+// round trip: threads past the grid's edge read the nearest cell and store
+// nothing, and at the low edges the neighbour offset is 0 (the difference is
+// dropped).
 template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC = false>
 __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile, int64_t q) {
   using T = typename FS::R;
   const Dims d = a.d;
-  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
-  const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
-  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
-  const int k0 = tz * bz + (threadIdx.x & (bz - 1)), j0 = ty * by + (threadIdx.x >> bzs);
-  const bool act = k0 < d.nz && j0 < d.ny;
-  const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
-  const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
+  int j, k, i0, i1;
+  bool act;
+  tile_column(a, tile, j, k, i0, i1, act);
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz;
   const int64_t zm = k > 0 ? 1 : 0, ym = j > 0 ? sy : 0;
   const T zero = T(0);
@@ -606,8 +644,7 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
 // tiles per x-chunk and the number of x-chunks of a single-grid launch
 template <class FS, class MS>
 __device__ __forceinline__ void chunk_counts(const MainArgs<FS, MS>& a, unsigned& per_chunk, unsigned& nchunks) {
-  const int bz = 1 << a.bz_shift;
-  per_chunk = (unsigned)(((a.d.nz + bz - 1) / bz) * ((a.d.ny + a.by - 1) / a.by));
+  per_chunk = plane_tiles(a.d, a.bz_shift, a.by, a.flat);
   nchunks = (unsigned)((a.d.nx + a.cx - 1) / a.cx);
 }
 
@@ -645,13 +682,9 @@ template <class FS, class MS, bool NT>
 __device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__ a, unsigned tile) {
   using T = typename FS::R;
   const Dims d = a.d;
-  const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
-  const int ntz = (d.nz + bz - 1) >> bzs, nty = (d.ny + by - 1) / by;
-  const int tz = tile % ntz, ty = (tile / ntz) % nty, tx = tile / (ntz * nty);
-  const int k0 = tz * bz + (threadIdx.x & (bz - 1)), j0 = ty * by + (threadIdx.x >> bzs);
-  const bool act = k0 < d.nz && j0 < d.ny;
-  const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
-  const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
+  int j, k, i0, i1;
+  bool act;
+  tile_column(a, tile, j, k, i0, i1, act);
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz;
   const int64_t zp = k < d.nz - 1 ? 1 : 0, yp = j < d.ny - 1 ? sy : 0;
   const T zero = T(0);
