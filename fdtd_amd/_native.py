@@ -53,6 +53,7 @@ _SIGNATURES = {
     "fdtd_batch_work_items": ([_vp, _pi64], _c_int),
     "fdtd_batch_run": ([_vp, _c_i64, _c_i64, _vp], _c_int),
     "fdtd_batch_run_graph": ([_vp, _c_i64, _c_i64, _c_i64, _vp], _c_int),
+    "fdtd_batch_run_static": ([_c_int, ctypes.POINTER(_vp), _c_int, _c_int, _c_int, _c_i64, _c_i64, _vp], _c_int),
     "fdtd_adjoint_psi_size": ([_vp, _pi64], _c_int),
     "fdtd_adjoint_create": ([_vp, _vp, _vp, _vp, _vp, _vp, _pi32, _pi32, _vp, ctypes.POINTER(_vp)], _c_int),
     "fdtd_adjoint_destroy": ([_vp], _c_int),
@@ -67,6 +68,14 @@ def bench_copy(dst, src, nbytes, reps, stream):
     ms = ctypes.c_float()
     check(lib().fdtd_bench_copy(dst.data_ptr(), src.data_ptr(), int(nbytes), int(reps), stream, ctypes.byref(ms)))
     return ms.value
+
+
+def batch_run_static(plans, block_threads, cx, workgroups, q0, nsteps, stream):
+    """fdtd_batch_run_static over finalized Plans: a batched run whose tables
+    live in the library's static device array (no device allocation)."""
+    arr = (_vp * len(plans))(*[p._h.value for p in plans])
+    check(lib().fdtd_batch_run_static(len(plans), arr, int(block_threads), int(cx), int(workgroups), int(q0),
+                                      int(nsteps), stream))
 
 
 def exported_symbols():

@@ -8,6 +8,7 @@
 #include <cstdint>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -729,6 +730,34 @@ BatchTables image_tables(const BatchImage& img, void* base) {
                      reinterpret_cast<const int32_t*>(c + img.oI), img.total, reinterpret_cast<int64_t*>(c + img.oQ)};
 }
 
+// ----------------------------------------------- batch without allocation
+// fdtd_batch_run_static keeps a batch's tables in this array instead of in
+// memory it allocates. Each GPU the library runs on holds its own copy of
+// the array until the process ends. In host code `scratch` names no device
+// address: the address of the current device's copy comes from
+// hipGetSymbolAddress (static_scratch below), and only that is passed on.
+constexpr size_t kStaticBatchBytes = FDTD_STATIC_BATCH_BYTES;
+static __device__ __align__(256) unsigned char scratch[kStaticBatchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array (device `dev` must be
+// current); looked up once per device.
+int static_scratch(int dev, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (dev < 0 || dev >= kMaxDevices) return fail(-1, "device index out of range for the static tables");
+  if (!g_scratch_ptr[dev]) {
+    void* ptr = nullptr;
+    FDTD_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    if (!ptr) return fail(-3, "hipGetSymbolAddress returned no address for the static tables");
+    g_scratch_ptr[dev] = ptr;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+
 int batch_room(const std::vector<FdtdPlan*>& plans, int64_t q0, int64_t nsteps) {
   for (FdtdPlan* p : plans)
     if (int rc = check_step_room(p, q0, nsteps, false, false)) return rc;
@@ -1393,6 +1422,40 @@ int fdtd_batch_run_graph(FdtdBatch* b, int64_t q0, int64_t nsteps, int64_t steps
   FDTD_HIP(hipEventRecord(b->ev_out, b->gstream));
   FDTD_HIP(hipStreamWaitEvent(caller, b->ev_out, 0));
   batch_advance(b->plans, nsteps);
+  return 0;
+}
+
+int fdtd_batch_run_static(int n, FdtdPlan* const* plans, int block_threads, int cx, int workgroups, int64_t q0,
+                          int64_t nsteps, void* stream) {
+  if (n < 1 || !plans) return fail(-1, "a batch needs at least one plan");
+  if (block_threads != 256 && block_threads != 512 && block_threads != 1024)
+    return fail(-1, "block_threads must be 256, 512 or 1024");
+  if (cx < 1 || workgroups < 0) return fail(-1, "bad launch shape");
+  if (nsteps < 0) return fail(-1, "negative step count");
+  for (int i = 0; i < n; ++i)
+    if (int rc = batch_check_plan(plans[i], plans[0])) return rc;
+  const std::vector<FdtdPlan*> pl(plans, plans + n);
+  if (int rc = batch_room(pl, q0, nsteps)) return rc;
+  if (int rc = select_device(pl[0])) return rc;
+  BatchImage img;
+  if (int rc = batch_image(pl, block_threads, cx, img)) return rc;
+  if (img.bytes.size() > kStaticBatchBytes)
+    return fail(-6, "the batch's tables need " + std::to_string(img.bytes.size()) + " bytes, more than the " +
+                        std::to_string(kStaticBatchBytes) +
+                        " of the static tables (FDTD_STATIC_BATCH_BYTES); use fdtd_batch_create for this batch");
+  void* base = nullptr;
+  if (int rc = static_scratch(pl[0]->device, &base)) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  // Stream-ordered after the work already queued on `stream` (an earlier
+  // call's kernels may still read the tables); the wait also keeps the host
+  // image alive until the copy has read it.
+  FDTD_HIP(hipMemcpyAsync(base, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice, st));
+  FDTD_HIP(hipStreamSynchronize(st));
+  const BatchTables t = image_tables(img, base);
+  hipLaunchKernelGGL(k_set_time, dim3(1), dim3(64), 0, st, t.q, q0);
+  const int groups = (workgroups > 0 && workgroups < img.total) ? workgroups : (int)img.total;
+  if (int rc = batch_launch_tables(pl[0], t, img.objs, groups, block_threads, nsteps, st)) return rc;
+  batch_advance(pl, nsteps);
   return 0;
 }
 

@@ -200,6 +200,27 @@ int fdtd_batch_run(FdtdBatch* batch, int64_t q0, int64_t nsteps, void* stream);
  * nsteps is launched directly. */
 int fdtd_batch_run_graph(FdtdBatch* batch, int64_t q0, int64_t nsteps, int64_t steps_per_graph, void* stream);
 
+/* Maximum bytes of batch tables that fdtd_batch_run_static can hold. */
+#define FDTD_STATIC_BATCH_BYTES (4u << 20)
+
+/* No counterpart in the reference. For callers that cannot allocate device
+ * memory (harnesses that forbid hipMalloc, tight serving loops): the same
+ * work as fdtd_batch_create + fdtd_batch_set_launch + fdtd_batch_run over
+ * the n finalized plans, and the same results, with the batch's tables
+ * (per-grid arguments, work list, time index) kept in a device array of
+ * FDTD_STATIC_BATCH_BYTES that the library holds on each GPU it runs on
+ * until the process ends, instead of in memory it allocates. No device
+ * memory is allocated or freed. The tables of a launch shape that need more
+ * than FDTD_STATIC_BATCH_BYTES are refused (error -6); the 64-grid,
+ * 132-million-cell batch needs about 2.7 MB at 256 threads and 1 plane.
+ * The call copies the tables on `stream` and waits for that stream (work
+ * queued on it before, then the copy), so it is not for graph capture.
+ * Every call on a GPU shares that GPU's array and leaves its tables there:
+ * calls that overlap on one GPU (other host threads, or other streams
+ * whose work is still running) race. */
+int fdtd_batch_run_static(int n, FdtdPlan* const* plans, int block_threads, int cx, int workgroups,
+                          int64_t q0, int64_t nsteps, void* stream);
+
 /* ---- Differentiation (reverse mode) -------------------------------------
  * The reference differentiates through its torch backend (fdtd/backend.py:
  * 207-361 with autograd enabled): torch records every array operation of

@@ -18,8 +18,8 @@ the device library gets. Prints one JSON line per case: {"case", "ok",
                       the fields against the fixture <= 1e-7
   batch:<seed>        grids of uneven shapes stepped by fdtd_batch_run
                       (direct launches, graph replays and a persistent
-                      launch), bit for bit against the same grids stepped
-                      alone
+                      launch) and by fdtd_batch_run_static, bit for bit
+                      against the same grids stepped alone
 """
 
 import ctypes
@@ -199,15 +199,19 @@ def case_batch(seed):
         g.run(steps, progress_bar=False)
     want = _state(alone)
     errs = []
-    for mode, launch in (("direct", (256, 1, 0)), ("graph", (256, 2, 0)), ("persistent", (512, 1, 3))):
+    for mode, launch in (("direct", (256, 1, 0)), ("graph", (256, 2, 0)), ("persistent", (512, 1, 3)),
+                         ("static", (256, 2, 0))):
         grids = _batch_grids(seed)
         plans = [g._prepare(steps, steps, steps) for g in grids]
-        b = _native.Batch(plans)
-        b.set_launch(*launch)
-        if mode == "graph":
-            b.run_graph(0, steps, 4, None)
+        if mode == "static":
+            _native.batch_run_static(plans, *launch, 0, steps, None)
         else:
-            b.run(0, steps, None)
+            b = _native.Batch(plans)
+            b.set_launch(*launch)
+            if mode == "graph":
+                b.run_graph(0, steps, 4, None)
+            else:
+                b.run(0, steps, None)
         for i, (x, y) in enumerate(zip(_state(grids), want)):
             if not torch.equal(x, y):
                 errs.append(f"{mode} {launch}: state tensor {i} differs")
