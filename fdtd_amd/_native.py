@@ -13,6 +13,7 @@ LIB_PATH = os.path.join(_HERE, "libfdtd_hip.so")
 
 F64, F32, F32C, C128, C64 = 0, 1, 2, 3, 4
 OBJ_PLAIN, OBJ_ABSORBING, OBJ_ANISOTROPIC = 0, 1, 2
+OBJ_REAL_ARRAYS = 16  # flag: the object's arrays are real in the reference
 SRC_POINT, SRC_LINE, SRC_PLANE = 0, 1, 2
 
 _lib = None

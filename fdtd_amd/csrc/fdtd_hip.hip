@@ -42,6 +42,7 @@ struct FaceHost {
 };
 struct ObjHost {
   int kind, x0, x1, y0, y1, z0, z1;
+  int real;  // FDTD_OBJ_REAL_ARRAYS was given
   const void* inv;
   const void* absorb;
 };
@@ -875,15 +876,15 @@ int fdtd_plan_add_periodic(FdtdPlan* p, int axis) {
 int fdtd_plan_add_object(FdtdPlan* p, int kind, int x0, int x1, int y0, int y1, int z0, int z1, const void* inv,
                          const void* absorption) {
   if (!p) return fail(-1, "null plan");
+  const int real = (kind & FDTD_OBJ_REAL_ARRAYS) ? 1 : 0;
+  kind &= ~FDTD_OBJ_REAL_ARRAYS;
   if (kind < FDTD_OBJ_PLAIN || kind > FDTD_OBJ_ANISOTROPIC) return fail(-1, "unknown object kind");
   if (x0 < 0 || y0 < 0 || z0 < 0 || x1 > p->d.nx || y1 > p->d.ny || z1 > p->d.nz || x0 >= x1 || y0 >= y1 ||
       z0 >= z1)
     return fail(-1, "object box outside the grid or empty");
   if (!inv) return fail(-1, "null object permittivity");
   if (kind == FDTD_OBJ_ABSORBING && !absorption) return fail(-1, "null absorption factor");
-  if (kind != FDTD_OBJ_PLAIN && is_complex(p))
-    return fail(-1, "absorbing and anisotropic objects are not supported on the complex-field path");
-  p->objs.push_back({kind, x0, x1, y0, y1, z0, z1, inv, absorption});
+  p->objs.push_back({kind, x0, x1, y0, y1, z0, z1, real, inv, absorption});
   p->finalized = false;
   return 0;
 }
@@ -1129,7 +1130,7 @@ int fdtd_plan_finalize(FdtdPlan* p) {
   std::memcpy(host.data() + o_faces, ordered.data(), nf * sizeof(FaceDev));
   for (size_t i = 0; i < no; ++i) {
     const ObjHost& o = p->objs[i];
-    ObjDev od{o.kind, o.x0, o.x1, o.y0, o.y1, o.z0, o.z1, 0, o.inv, o.absorb};
+    ObjDev od{o.kind, o.x0, o.x1, o.y0, o.y1, o.z0, o.z1, o.real, o.inv, o.absorb};
     std::memcpy(host.data() + o_objs + i * sizeof(ObjDev), &od, sizeof(ObjDev));
   }
   for (size_t i = 0; i < ns; ++i) {

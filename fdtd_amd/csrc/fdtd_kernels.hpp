@@ -61,7 +61,8 @@ struct FaceDev {
 };
 
 struct ObjDev {
-  int32_t kind, x0, x1, y0, y1, z0, z1, pad;
+  int32_t kind, x0, x1, y0, y1, z0, z1;
+  int32_t real;        // the reference's arrays of this object are real (see absorb_ratio)
   const void* inv;     // in the plan's object type
   const void* absorb;
 };
@@ -138,8 +139,10 @@ __device__ __forceinline__ void st_stream(T* p, T v) {
 // to (x, 0), which makes every real-times-complex product exactly the
 // componentwise one; sums and differences are componentwise. cplx<S> does
 // the same, so the complex path matches the reference bit for bit as well.
-// Division (Smith's method) exists only so that every kernel instantiates;
-// the host refuses the components that would divide complex values.
+// Division is numpy's for complex values (Smith's method, with its
+// zero-divisor branch); a real operand of a division is promoted to (x, 0)
+// there as well, which is NOT the componentwise quotient. Absorbing objects
+// divide (objects.py:215-221).
 template <typename S>
 struct cplx {
   S re, im;
@@ -154,6 +157,7 @@ struct cplx {
   }
   friend __host__ __device__ cplx operator/(cplx a, cplx b) {
     if (fabs(b.re) >= fabs(b.im)) {
+      if (b.re == S(0) && b.im == S(0)) return cplx(a.re / fabs(b.re), a.im / fabs(b.re));
       const S rat = b.im / b.re, scl = S(1) / (b.re + b.im * rat);
       return cplx((a.re + a.im * rat) * scl, (a.im - a.re * rat) * scl);
     }
@@ -412,6 +416,41 @@ __device__ __forceinline__ void pml_cell(const FaceDev* __restrict__ faces, int 
                    v[2]);
 }
 
+// E factor (1 - f) / (1 + f) of an absorbing object (objects.py:215). With
+// complex fields the reference divides in the dtype of f: complex (numpy's
+// division) when the object's permittivity is complex, real when it is real
+// (its arrays stay real on complex fields), and the real quotient then
+// multiplies E as (r, 0).
+template <typename T>
+__device__ __forceinline__ T absorb_ratio(T f, int) {
+  return (T(1) - f) / (T(1) + f);
+}
+template <typename S>
+__device__ __forceinline__ cplx<S> absorb_ratio(cplx<S> f, int real) {
+  if (real) return cplx<S>((S(1) - f.re) / (S(1) + f.re), S(0));
+  return (cplx<S>(1) - f) / (cplx<S>(1) + f);
+}
+
+// Row of the anisotropic 3x3 product inv @ curl (objects.py:262-269, numpy's
+// einsum "ijk,ikl->ijl"). Real: the association (m0 c0 + m2 c2) + m1 c1
+// numpy's einsum uses for this 3-term contraction. Complex: einsum's complex
+// loop, products without fused multiply-adds (re = ar br - ai bi,
+// im = ar bi + ai br) summed in order; a real inverse on complex fields is
+// promoted to (x, 0) by einsum, which the same loop covers.
+template <typename T>
+__device__ __forceinline__ T aniso_row(T m0, T m1, T m2, T c0, T c1, T c2) {
+  return (m0 * c0 + m2 * c2) + m1 * c1;
+}
+template <typename S>
+__device__ __forceinline__ cplx<S> einsum_product(cplx<S> a, cplx<S> b) {
+  return cplx<S>(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+template <typename S>
+__device__ __forceinline__ cplx<S> aniso_row(cplx<S> m0, cplx<S> m1, cplx<S> m2, cplx<S> c0, cplx<S> c1,
+                                             cplx<S> c2) {
+  return (einsum_product(m0, c0) + einsum_product(m1, c1)) + einsum_product(m2, c2);
+}
+
 // Object updates of one cell, registration order (objects.py:118-269); the
 // object arrays are stored as O and computed in T.
 template <typename T, typename O>
@@ -431,16 +470,15 @@ __device__ __forceinline__ void objects_update_E(const ObjDev* __restrict__ objs
 #pragma unroll
       for (int c = 0; c < 3; ++c) {
         const T f = at(fa, lo * 3 + c);
-        e[c] = e[c] * ((T(1) - f) / (T(1) + f));
+        e[c] = e[c] * absorb_ratio(f, ob.real);
         e[c] = e[c] + ((sc * at(inv, lo * 3 + c)) * curl[c]) / (T(1) + f);
       }
     } else {
       const O* const m = inv + lo * 9;
       T v[3];
 #pragma unroll
-      // the association numpy's einsum uses for this 3-term contraction
       for (int r = 0; r < 3; ++r)
-        v[r] = (at(m, 3 * r) * curl[0] + at(m, 3 * r + 2) * curl[2]) + at(m, 3 * r + 1) * curl[1];
+        v[r] = aniso_row(at(m, 3 * r), at(m, 3 * r + 1), at(m, 3 * r + 2), curl[0], curl[1], curl[2]);
 #pragma unroll
       for (int r = 0; r < 3; ++r) e[r] = e[r] + sc * v[r];
     }

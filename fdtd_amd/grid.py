@@ -317,9 +317,6 @@ class Grid:
         self._E = self._E.to(cdt)
         self._H = self._H.to(cdt)
         self.complex = True
-        for obj in self.objects:
-            if obj._kind != _native.OBJ_PLAIN:
-                raise NotImplementedError(f"{type(obj).__name__} is not supported on the complex-field path")
         for det in self.detectors:
             det._promote(cdt)
         for boundary in self.boundaries:

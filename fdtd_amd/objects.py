@@ -51,8 +51,8 @@ class Object:
         self.grid = grid
         self.grid.objects.append(self)
         complex_perm = _is_complex(self.permittivity)
-        if self._kind != _native.OBJ_PLAIN and (complex_perm or grid.complex):
-            raise NotImplementedError(f"{type(self).__name__} is not supported on the complex-field path")
+        # the reference keeps a real object's arrays real on complex fields
+        self._real_arrays = not complex_perm
         if complex_perm:
             # objects.py:51-52: E, H and the CPML state registered so far
             # become complex before the object computes its inverse
@@ -135,7 +135,8 @@ class Object:
     def _add_to_plan(self, plan):
         inv, absorb = self._arrays()
         box = ((self.x.start, self.x.stop), (self.y.start, self.y.stop), (self.z.start, self.z.stop))
-        plan.add_object(self._kind, box, inv, absorb)
+        kind = self._kind | (_native.OBJ_REAL_ARRAYS if self._real_arrays else 0)
+        plan.add_object(kind, box, inv, absorb)
 
     def update_E(self, curl_H):
         raise NotImplementedError("object updates run inside the fused HIP step")
@@ -175,9 +176,12 @@ class AbsorbingObject(Object):
             cond = cond[..., None]
         cond = np.broadcast_to(cond, self._inv_host.shape)
         self.conductivity = torch.from_numpy(np.ascontiguousarray(cond)).to(grid._E.device)
-        # objects.py:198-205, evaluated left to right in the field dtype
+        # objects.py:198-205, evaluated left to right in the dtype of the
+        # object's inverse permittivity (complex for a complex permittivity)
         f = 0.5 * grid.courant_number * self._inv_host * cond * grid.grid_spacing * const.eta0
-        self.absorption_factor = torch.from_numpy(np.ascontiguousarray(f.astype(npdt))).to(grid._E.device)
+        f = f.astype(self._inv_host.dtype)
+        self._real_arrays = not np.iscomplexobj(f)
+        self.absorption_factor = torch.from_numpy(np.ascontiguousarray(f)).to(grid._E.device)
 
     def _arrays(self):
         shape = (self.Nx, self.Ny, self.Nz, 3)

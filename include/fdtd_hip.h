@@ -24,10 +24,11 @@
  *   - FDTD_C128 / FDTD_C64: the complex-field path of the reference
  *     (fdtd/grid.py:353-356): E, H, CPML state and object inverse
  *     permittivities complex (interleaved re, im; complex128 / complex64),
- *     grid materials real (float64 / float32), detector traces complex.
- *     Absorbing and anisotropic objects, soft sources and current detectors
- *     are refused on this path; so are the material palette, batching and
- *     the adjoint.
+ *     grid materials real (float64 / float32), detector traces complex;
+ *     object arrays complex, holding (x, 0) for an object whose arrays are
+ *     real in the reference (FDTD_OBJ_REAL_ARRAYS). Soft sources and current
+ *     detectors are refused on this path; so are the material palette,
+ *     batching and the adjoint.
  *   - Every function returns 0 on success and a negative code on error;
  *     fdtd_last_error() then describes the failure. No C++ exception crosses
  *     the ABI.
@@ -46,6 +47,11 @@ typedef struct FdtdPlan FdtdPlan;
 
 enum { FDTD_F64 = 0, FDTD_F32 = 1, FDTD_F32C = 2, FDTD_C128 = 3, FDTD_C64 = 4 };
 enum { FDTD_OBJ_PLAIN = 0, FDTD_OBJ_ABSORBING = 1, FDTD_OBJ_ANISOTROPIC = 2 };
+/* Flag or-ed into an object kind: the reference holds this object's arrays
+ * as real ones (a real permittivity and conductivity). It matters on the
+ * complex-field path only, where an absorbing object's factor
+ * (1 - f) / (1 + f) is then a real division (fdtd/objects.py:215). */
+enum { FDTD_OBJ_REAL_ARRAYS = 16 };
 enum { FDTD_SRC_POINT = 0, FDTD_SRC_LINE = 1, FDTD_SRC_PLANE = 2, FDTD_SRC_SOFT = 3 };
 
 /* Version string of the library build. */
@@ -101,7 +107,8 @@ int fdtd_plan_add_periodic(FdtdPlan* plan, int axis);
 /* Object over the box [x0,x1) x [y0,y1) x [z0,z1) (fdtd/objects.py:24-277).
  * inv: the object's inverse permittivity, (nx,ny,nz,3) for PLAIN and
  * ABSORBING, (nx,ny,nz,3,3) for ANISOTROPIC. absorption: (nx,ny,nz,3)
- * absorption factors for ABSORBING (objects.py:198-205), else NULL. */
+ * absorption factors for ABSORBING (objects.py:198-205), else NULL. kind may
+ * carry FDTD_OBJ_REAL_ARRAYS. */
 int fdtd_plan_add_object(FdtdPlan* plan, int kind, int x0, int x1, int y0, int y1,
                          int z0, int z1, const void* inv, const void* absorption);
 

@@ -14,6 +14,7 @@ the device library gets. Prints one JSON line per case: {"case", "ok",
                       bit against the fixture
   random:<seed>       a random scenario against the oracle, float64 (bit for
                       bit unless it has an anisotropic object: 1e-12)
+  random_complex:<seed>  the same on the complex-field path
   compensated:<name>  a golden scenario in hip.float32c: relative L2 error of
                       the fields against the fixture <= 1e-7
   batch:<seed>        grids of uneven shapes stepped by fdtd_batch_run
@@ -123,11 +124,11 @@ def case_fixture(name):
     return [e for e in errs if e]
 
 
-def case_random(seed):
+def case_random(seed, make=None):
     from oracle import fdtd_oracle as orc
 
     fdtd.set_backend("hip.float64")
-    sc = scenarios.random_scenario(int(seed))
+    sc = (make or scenarios.random_scenario)(int(seed))
     sim, _ = scenarios.build_oracle(orc, sc)
     scenarios.run_program(sim, sc)
     grid, comps = scenarios.build_fdtd(fdtd, sc, to_array=torch.from_numpy)
@@ -139,6 +140,10 @@ def case_random(seed):
         errs.append(_same(_np(d.trace("E")).reshape(np.array(o.E).shape), np.array(o.E), "detector E", bitwise))
         errs.append(_same(_np(d.trace("H")).reshape(np.array(o.H).shape), np.array(o.H), "detector H", bitwise))
     return [e for e in errs if e]
+
+
+def case_random_complex(seed):
+    return case_random(seed, scenarios.random_complex_scenario)
 
 
 def case_compensated(name):
@@ -218,7 +223,8 @@ def case_batch(seed):
     return errs
 
 
-CASES = {"fixture": case_fixture, "random": case_random, "compensated": case_compensated, "batch": case_batch}
+CASES = {"fixture": case_fixture, "random": case_random, "random_complex": case_random_complex,
+         "compensated": case_compensated, "batch": case_batch}
 
 
 def main(argv):

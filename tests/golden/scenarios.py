@@ -298,9 +298,56 @@ def scenario_complex_object():
     )
 
 
+def scenario_complex_absorbing():
+    """Absorbing and anisotropic objects on the complex-field path
+    (objects.py:163-277 after grid.py:353-356): a plain Object with a complex
+    permittivity promotes the fields; then an AbsorbingObject with a complex
+    permittivity (complex absorption factor: numpy's complex division each
+    step), an AbsorbingObject with a real permittivity (real arrays on complex
+    fields: a real division, then mixed products), an AnisotropicObject
+    whose inverse is overwritten by a per-cell complex 3x3 matrix (complex
+    einsum), and one with a real permittivity (real 3x3 on complex fields).
+    PML on four faces (registered first), a pulsed point source, a line
+    source, a line detector and random real initial fields."""
+    rs = np.random.RandomState(83)
+    s = slice
+    shape = (20, 16, 14)
+    aniso_shape = (4, 3, 4)
+    full = _spd_inverse_field(rs, aniso_shape, 1.5, 4.0).astype(np.complex128)
+    full += 1j * rs.uniform(-0.01, 0.01, size=full.shape)
+    # per cell, so that the real division of the real object's factor is
+    # seen on many values (a quotient and a product by the reciprocal agree
+    # on about half of them)
+    real_perm = rs.uniform(1.5, 3.0, size=(4, 4, 4))
+    return dict(
+        name="complex_absorbing",
+        shape=shape,
+        grid_spacing=50e-9,
+        courant_number=None,
+        permittivity=1.0,
+        permeability=1.0,
+        init={"E": rs.uniform(-1, 1, size=shape + (3,)), "H": rs.uniform(-1, 1, size=shape + (3,))},
+        components=[
+            ("PML", dict(), (s(0, 3), s(None), s(None)), {}),
+            ("PML", dict(), (s(-3, None), s(None), s(None)), {}),
+            ("PML", dict(), (s(None), s(0, 3), s(None)), {}),
+            ("PML", dict(), (s(None), s(None), s(-3, None)), {}),
+            ("Object", dict(permittivity=2.2 + 0.03j), (s(4, 8), s(4, 9), s(3, 7)), {}),
+            ("AbsorbingObject", dict(permittivity=3.0 + 0.04j, conductivity=2e4), (s(9, 14), s(3, 7), s(4, 9)), {}),
+            ("AbsorbingObject", dict(permittivity=real_perm, conductivity=5e4), (s(12, 16), s(9, 13), s(2, 6)), {}),
+            ("AnisotropicObject", dict(permittivity=2.0), (s(5, 9), s(10, 13), s(7, 11)), {"inverse_permittivity": full}),
+            ("AnisotropicObject", dict(permittivity=1.6), (s(14, 17), s(4, 8), s(9, 12)), {}),
+            ("PointSource", dict(period=10, pulse=True), (10, 8, 7), {}),
+            ("LineSource", dict(period=14), (s(4, 16), 12, 9), {}),
+            ("LineDetector", dict(name="det"), (s(3, 17), 8, 9), {}),
+        ],
+        program=[("step", 40)],
+    )
+
+
 def all_scenarios():
     out = [scenario_c1(), scenario_g2(), scenario_manual(), scenario_float_keys(), scenario_1d(),
-           scenario_soft_source(), scenario_complex_object()]
+           scenario_soft_source(), scenario_complex_object(), scenario_complex_absorbing()]
     for axis in range(3):
         for first in (True, False):
             out.append(_periodic_scenario(f"periodic_{'xyz'[axis]}_{'first' if first else 'last'}", axis, first))
@@ -420,25 +467,32 @@ def random_complex_scenario(seed, steps=15):
     """random_scenario(seed) moved onto the complex-field path: periodic
     boundaries dropped (the reference cannot promote them), the PML faces
     registered first (a PML registered after the promotion fails there),
-    every object a plain Object, the first of them with a complex
-    permittivity (scalar or per cell, imaginary part <= 0.05: the complex
-    leap-frog grows otherwise), one added when the scenario has none."""
+    the first object with a complex permittivity (scalar or per cell,
+    imaginary part <= 0.05: the complex leap-frog grows otherwise; an
+    overwritten anisotropic inverse gets imaginary parts <= 0.01), a plain
+    Object added first when the scenario has no object; the other objects,
+    of every kind, keep real arrays on the complex fields."""
     sc = random_scenario(seed, steps)
     rs = np.random.RandomState(5000 + seed)
     comps = [c for c in sc["components"] if c[0] != "PeriodicBoundary"]
     pmls = [c for c in comps if c[0] == "PML"]
-    objs = [("Object", dict(permittivity=c[1]["permittivity"]), c[2], {}) for c in comps if c[0].endswith("Object")]
+    objs = [c for c in comps if c[0].endswith("Object")]
     rest = [c for c in comps if c[0] != "PML" and not c[0].endswith("Object")]
     if not objs:
         shape = sc["shape"]
         key = tuple(slice(n // 3, max(n // 3 + 1, 2 * n // 3)) for n in shape)
         objs.append(("Object", dict(permittivity=2.0), key, {}))
-    perm = objs[0][1]["permittivity"]
+    kind, params, key, over = objs[0]
+    params, over = dict(params), dict(over)
+    perm = params["permittivity"]
     if np.ndim(perm) == 0:
-        perm = complex(perm, float(rs.uniform(0.005, 0.05)))
+        params["permittivity"] = complex(perm, float(rs.uniform(0.005, 0.05)))
     else:
-        perm = perm + 1j * rs.uniform(0.005, 0.05, size=np.shape(perm))
-    objs[0] = ("Object", dict(permittivity=perm), objs[0][2], {})
+        params["permittivity"] = perm + 1j * rs.uniform(0.005, 0.05, size=np.shape(perm))
+    if "inverse_permittivity" in over:
+        inv = over["inverse_permittivity"]
+        over["inverse_permittivity"] = inv + 1j * rs.uniform(-0.01, 0.01, size=inv.shape)
+    objs[0] = (kind, params, key, over)
     sc["components"] = pmls + objs + rest
     sc["name"] = f"random_complex_{seed}"
     return sc

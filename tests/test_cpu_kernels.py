@@ -4,9 +4,10 @@ and AddressSanitizer against a host stand-in for the HIP runtime
 
 Every golden scenario the reference wrote is replayed through the package's
 own Python layer and must match bit for bit (fields, CPML state, detector and
-source traces), as on the GPU; random scenarios match the oracle; the
-compensated form stays within its bound; batched runs (direct, graph replay,
-persistent, and the static-table entry point) equal the grids stepped alone.
+source traces), as on the GPU; random scenarios, real and complex, match the
+oracle; the compensated form stays within its bound; batched runs (direct,
+graph replay, persistent, and the static-table entry point) equal the grids
+stepped alone.
 Both tile shapes of the main kernels run: the default (flat tiles for these
 grids, whose nz is not a multiple of 64) and row tiles forced with
 FDTD_HIP_FLAT=0. Any out-of-bounds access aborts the child process with an
@@ -29,6 +30,7 @@ def _cases():
 
     out = [f"fixture:{sc['name']}" for sc in scenarios.all_scenarios()]
     out += [f"random:{s}" for s in range(6)]
+    out += [f"random_complex:{s}" for s in range(6)]
     out += ["compensated:g2", "compensated:periodic_x_first", "batch:1", "batch:2"]
     return out
 

@@ -421,9 +421,12 @@ def test_complex_fields_pml_registered_late_fails_like_the_reference():
 
 @pytest.mark.parametrize("seed", range(6))
 def test_complex_random_scenarios_match_oracle(seed):
-    """Random configurations on the complex-field path (complex128) against
-    the oracle's numpy complex arithmetic, bit for bit; the energy plane of
-    visualize() computed on the device against numpy's."""
+    """Random configurations on the complex-field path (complex128), objects
+    of every kind (absorbing: numpy's complex division; anisotropic: its
+    complex einsum), against the oracle's numpy complex arithmetic, bit for
+    bit (anisotropic cases within 1e-12: einsum's association on the host
+    running the oracle decides there); the energy plane of visualize()
+    computed on the device against numpy's."""
     sc = scenarios.random_complex_scenario(seed)
     _say(f"[gpu] random complex scenario {seed}: shape {sc['shape']}, {[c[0] for c in sc['components']]}")
     sim, ocomps = scenarios.build_oracle(orc, sc)
@@ -432,12 +435,13 @@ def test_complex_random_scenarios_match_oracle(seed):
     assert grid.complex
     scenarios.run_program(grid, sc)
     torch.cuda.synchronize()
-    _assert_same(_np(grid.E), sim.E, "E")
-    _assert_same(_np(grid.H), sim.H, "H")
+    bitwise = not any(k[0] == "AnisotropicObject" for k in sc["components"])
+    _assert_same(_np(grid.E), sim.E, "E", bitwise)
+    _assert_same(_np(grid.H), sim.H, "H", bitwise)
     dets = [c for c in comps if isinstance(c, (fdtd.LineDetector, fdtd.BlockDetector))]
     for d, o in zip(dets, sim.detectors):
-        _assert_same(_np(d.trace("E")).reshape(np.array(o.E).shape), np.array(o.E), "detector E")
-        _assert_same(_np(d.trace("H")).reshape(np.array(o.H).shape), np.array(o.H), "detector H")
+        _assert_same(_np(d.trace("E")).reshape(np.array(o.E).shape), np.array(o.E), "detector E", bitwise)
+        _assert_same(_np(d.trace("H")).reshape(np.array(o.H).shape), np.array(o.H), "detector H", bitwise)
     z = sc["shape"][2] // 2
     if sc["shape"][0] > 1 and sc["shape"][1] > 1:
         ref = np.sum(sim.E ** 2 + sim.H ** 2, -1)[:, :, z]

@@ -208,7 +208,10 @@ def test_stepping_refuses_host_arrays():
 def test_complex_permittivity_promotes_fields_and_cpml_state():
     """objects.py:51-52 / grid.py:353-356: an Object with a complex
     permittivity makes E, H and the PML state registered before it complex;
-    periodic boundaries and absorbing objects are refused as documented."""
+    a later absorbing object with a real permittivity keeps real arrays (its
+    factor (1 - f)/(1 + f) is then a real division), one with a complex
+    permittivity gets a complex absorption factor; periodic boundaries are
+    refused as in the reference."""
     grid = fdtd.Grid((12, 10, 8), grid_spacing=50e-9)
     grid[0:2, :, :] = fdtd.PML()
     grid[4:7, 3:6, 2:5] = fdtd.Object(permittivity=2.0 + 0.1j)
@@ -217,8 +220,15 @@ def test_complex_permittivity_promotes_fields_and_cpml_state():
     inv = grid.objects[0].inverse_permittivity
     assert inv.dtype == torch.complex128
     assert inv[0, 0, 0, 0].item() == np.complex128(1) / np.complex128(2.0 + 0.1j)
-    with pytest.raises(NotImplementedError):
-        grid[8:10, 3:6, 2:5] = fdtd.AbsorbingObject(permittivity=1.5, conductivity=1e3)
+    assert grid.objects[0]._real_arrays is False
+    grid[8:10, 3:6, 2:5] = fdtd.AbsorbingObject(permittivity=1.5, conductivity=1e3)
+    real = grid.objects[1]
+    assert real._real_arrays and not real.absorption_factor.is_complex()
+    grid[1:3, 6:9, 2:5] = fdtd.AbsorbingObject(permittivity=1.5 + 0.02j, conductivity=1e3)
+    lossy = grid.objects[2]
+    assert not lossy._real_arrays and lossy.absorption_factor.dtype == torch.complex128
+    f = 0.5 * grid.courant_number * lossy._inv_host * 1e3 * grid.grid_spacing * fdtd.constants.eta0
+    assert np.array_equal(lossy.absorption_factor.numpy(), f)
     g2 = fdtd.Grid((12, 10, 8), grid_spacing=50e-9)
     g2[:, :, 0] = fdtd.PeriodicBoundary()
     with pytest.raises(AttributeError):

@@ -200,6 +200,64 @@ def test_complex_path_pins_numpys_complex_product(golden, monkeypatch):
     assert np.abs(sim.E - g["E"]).max() < 1e-12 * np.abs(g["E"]).max()
 
 
+def _planted_absorbing(ratio, quotient):
+    """An absorbing-object update with the E factor and the curl term's
+    division computed by the given functions."""
+    def update(self, curl):
+        s = self.sim
+        loc = (self.x, self.y, self.z)
+        if self.kind != "absorbing":
+            return orig_update(self, curl)
+        f = self.absorption_factor
+        s.E[loc] *= ratio(f)
+        s.E[loc] += quotient(s.courant_number * self.inverse_permittivity * curl[loc], 1 + f)
+    orig_update = orc._Object.update_E
+    return update
+
+
+def test_complex_absorbing_fixture_pins_numpys_division(golden, monkeypatch):
+    """The complex_absorbing fixture (written by the reference) fixes numpy's
+    complex division (Smith's method) for complex absorption factors and a
+    real division for an absorbing object whose arrays are real on complex
+    fields: the oracle (numpy) matches it bit for bit, and three plausible
+    restatements do not -- multiplying by the reciprocal, dividing the real
+    object's factor in complex arithmetic, and einsum products with fused
+    multiply-adds (numpy's ufunc product) in the anisotropic mat-vec."""
+    g = golden("complex_absorbing")
+    sc, sim, comps = _run("complex_absorbing")
+    kinds = [(o.kind, np.iscomplexobj(getattr(o, "absorption_factor", 0j))) for o in sim.objects]
+    assert ("absorbing", True) in kinds and ("absorbing", False) in kinds
+    scenarios.run_program(sim, sc)
+    assert np.array_equal(sim.E, g["E"]) and np.array_equal(sim.H, g["H"])
+
+    plants = {
+        "reciprocal": _planted_absorbing(lambda f: (1 - f) * (1 / (1 + f)), lambda a, b: a * (1 / b)),
+        "complex ratio of a real factor": _planted_absorbing(
+            lambda f: (1 - f.astype(np.complex128)) / (1 + f.astype(np.complex128)), lambda a, b: a / b),
+    }
+    for what, fn in plants.items():
+        monkeypatch.setattr(orc._Object, "update_E", fn)
+        sc, sim, comps = _run("complex_absorbing")
+        scenarios.run_program(sim, sc)
+        assert not np.array_equal(sim.E, g["E"]), what
+        assert np.abs(sim.E - g["E"]).max() < 1e-10 * np.abs(g["E"]).max(), what
+        monkeypatch.undo()
+
+    real_einsum = np.einsum
+
+    def fused_einsum(spec, m, v):  # ufunc products (fused multiply-add), same order
+        if spec != "ijk,ikl->ijl" or not np.iscomplexobj(m):
+            return real_einsum(spec, m, v)
+        v = v.astype(m.dtype)
+        return ((m[:, :, 0] * v[:, 0]) + (m[:, :, 1] * v[:, 1]) + (m[:, :, 2] * v[:, 2]))[..., None]
+
+    monkeypatch.setattr(orc.np, "einsum", fused_einsum)
+    sc, sim, comps = _run("complex_absorbing")
+    scenarios.run_program(sim, sc)
+    assert not np.array_equal(sim.E, g["E"])
+    assert np.abs(sim.E - g["E"]).max() < 1e-10 * np.abs(g["E"]).max()
+
+
 def test_complex_permittivity_with_periodic_boundary_fails_like_the_reference():
     sim = orc.Sim((8, 8, 8), grid_spacing=50e-9)
     sim.add("PeriodicBoundary", (slice(None), slice(None), 0))
