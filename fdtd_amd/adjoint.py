@@ -35,7 +35,7 @@ import numpy as np
 import torch
 
 from . import _native
-from .boundaries import PML
+from .boundaries import PML, PeriodicBoundary
 from .objects import AbsorbingObject, AnisotropicObject, Object
 
 # checkpointed mode: at most RECORD_BUFFERS record buffers of K steps (the
@@ -77,13 +77,17 @@ class IntensityGradient:
     """Objective J = sum_t sum_box |E|^2 of a T-step run from rest and its
     gradient with respect to the permittivity of the design object.
 
-    grid:        a HIP-backend grid with PML faces, plain or absorbing
-                 objects and point, line or plane sources (no periodic
-                 boundaries, no anisotropic objects, no soft sources, float64
-                 or float32 storage); detectors are ignored. A plane source
-                 assigns E and H at its cells (sources.py:476-486): those
-                 values do not depend on the state, so the transposed half
-                 steps zero the adjoint fields there.
+    grid:        a HIP-backend grid with PML faces (at most one per side),
+                 periodic boundaries registered after every PML face, plain,
+                 absorbing and anisotropic objects (an anisotropic object
+                 may not overlap an absorbing one) and point, line or plane
+                 sources (no soft sources), float64 or float32 storage;
+                 detectors are ignored. A plane source assigns E and H at its
+                 cells (sources.py:476-486): those values do not depend on
+                 the state, so the transposed half steps zero the adjoint
+                 fields there. A periodic boundary copies a plane after each
+                 half step (boundaries.py:184-219): its transpose adds the
+                 copied plane's adjoint to the source plane's and zeroes it.
     design:      a plain ``Object`` registered in ``grid``
     box:         the objective box, a grid index tuple (x, y, z)
     steps:       T
@@ -95,13 +99,22 @@ class IntensityGradient:
             raise NotImplementedError("gradients need real hip.float64 or hip.float32 fields")
         if type(design) is not Object or design.grid is not grid:
             raise ValueError("the design region must be a plain Object registered in the grid")
-        if grid.boundaries and any(not isinstance(b, PML) for b in grid.boundaries):
-            raise NotImplementedError("gradients support PML boundaries only (no periodic boundaries)")
+        seen_periodic = False
+        for b in grid.boundaries:
+            if isinstance(b, PeriodicBoundary):
+                seen_periodic = True
+            elif not isinstance(b, PML):
+                raise NotImplementedError(f"gradients do not support {type(b).__name__}")
+            elif seen_periodic:
+                raise NotImplementedError("gradients need every PML face registered before the periodic boundaries")
         for obj in grid.objects:
-            if isinstance(obj, AnisotropicObject):
-                raise NotImplementedError("gradients do not support anisotropic objects")
             if obj is not design and _overlap(obj, design):
                 raise ValueError("another object overlaps the design region")
+            if isinstance(obj, AnisotropicObject):
+                for other in grid.objects:
+                    if isinstance(other, AbsorbingObject) and _overlap(obj, other):
+                        raise NotImplementedError("gradients do not support an anisotropic object overlapping an "
+                                                  "absorbing one (the two updates do not compose)")
         for src in grid.sources:
             if type(src).__name__ == "SoftArbitraryPointSource":
                 raise NotImplementedError("soft sources are not supported by the gradient (a resistive one feeds "
@@ -150,12 +163,16 @@ class IntensityGradient:
         return p
 
     def _coefficients(self):
-        """(gcoef, dcoef) of the composed E update E' = dcoef E + gcoef curl H."""
+        """(gcoef, dcoef) of the composed E update E' = dcoef E + gcoef curl H
+        (the diagonal part: the library adds the transposes of the
+        anisotropic objects' 3x3 updates itself)."""
         g = self.grid
         S = g.courant_number
         gcoef = (S * g._inv_eps.to(torch.float64))
         dcoef = None
         for obj in g.objects:
+            if isinstance(obj, AnisotropicObject):
+                continue
             sl = (slice(None), obj.x, obj.y, obj.z)
             inv = obj.inverse_permittivity.to(torch.float64).permute(3, 0, 1, 2)
             if isinstance(obj, AbsorbingObject):
@@ -180,8 +197,7 @@ class IntensityGradient:
 
     def _new_plan(self):
         g = self.grid
-        if g._E.device.type != "cuda":
-            raise RuntimeError("gradients need a HIP device")
+        _require_device(g._E)
         code = _native.F64 if g.dtype == torch.float64 else _native.F32
         plan = _native.Plan(code, (g.Nx, g.Ny, g.Nz), g.courant_number, g._E, g._H, g._inv_eps, g._inv_mu)
         for comp in list(g.boundaries) + list(g.objects) + list(g.sources):
@@ -201,7 +217,8 @@ class IntensityGradient:
         g = self.grid
         out = [g._E, g._H]
         for b in g.boundaries:
-            out += [b._psiE, b._psiH]
+            if isinstance(b, PML):
+                out += [b._psiE, b._psiH]
         return out
 
     def _reset(self):
@@ -375,6 +392,11 @@ def _reduce_to(g, permittivity):
     if len(shape) == 0:
         return g.sum()
     raise ValueError(f"unsupported permittivity shape {shape}")
+
+
+def _require_device(t):
+    if t.device.type != "cuda":
+        raise RuntimeError("gradients need a HIP device")
 
 
 def _overlap(a, b):

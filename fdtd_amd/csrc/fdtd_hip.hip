@@ -161,6 +161,7 @@ struct FdtdAdjoint {
   void* d_sides = nullptr;
   int parE = 0, parH = 0;
   int cx = 0;  // x planes per workgroup of the transposed kernels (0: the plan's)
+  void* ybuf = nullptr;  // (3, N) Y buffer when the plan has anisotropic objects
 };
 
 namespace {
@@ -439,10 +440,21 @@ int adj_check_plan(const FdtdPlan* p) {
   for (const FaceHost& f : p->faces)
     if (++side[f.axis][f.high ? 1 : 0] > 1) return fail(-1, "adjoint needs at most one PML face per side");
   if (p->dtype != FDTD_F64 && p->dtype != FDTD_F32) return fail(-1, "adjoint needs FDTD_F64 or FDTD_F32 storage");
-  for (const Bnd& b : p->order)
-    if (b.periodic) return fail(-1, "adjoint does not support periodic boundaries");
+  // periodic boundaries are transposed as plane copies after the main
+  // kernels; a PML face registered after one would be a separate stage
+  for (const Bnd& b : p->stages)
+    if (!b.periodic) return fail(-1, "adjoint needs every PML face registered before the periodic boundaries");
+  // an anisotropic update adds to E; an absorbing one also scales it, so the
+  // two do not compose in either order when they overlap
+  auto overlap = [](const ObjHost& u, const ObjHost& v) {
+    return std::max(u.x0, v.x0) < std::min(u.x1, v.x1) && std::max(u.y0, v.y0) < std::min(u.y1, v.y1) &&
+           std::max(u.z0, v.z0) < std::min(u.z1, v.z1);
+  };
   for (const ObjHost& o : p->objs)
-    if (o.kind == FDTD_OBJ_ANISOTROPIC) return fail(-1, "adjoint does not support anisotropic objects");
+    if (o.kind == FDTD_OBJ_ANISOTROPIC)
+      for (const ObjHost& q : p->objs)
+        if (q.kind == FDTD_OBJ_ABSORBING && overlap(o, q))
+          return fail(-1, "adjoint does not support an anisotropic object overlapping an absorbing one");
   for (const SrcHost& s : p->srcs)
     if (s.kind == FDTD_SRC_SOFT) return fail(-1, "adjoint does not support soft sources");
   return 0;
@@ -458,6 +470,21 @@ void adj_mask_hard_sources(const FdtdPlan* p, T* lam, bool is_H, hipStream_t st)
     const int n = (int)s.cells.size();
     const int64_t base = (int64_t)(is_H ? s.compH : s.compE) * p->d.cs;
     hipLaunchKernelGGL(k_adj_zero<T>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, lam + base, s.d_cells, n);
+  }
+}
+
+// Transposes of the plan's periodic copies of one half step (the stages
+// after the main kernel; adj_check_plan admits periodic stages only), in
+// reverse registration order.
+template <typename T>
+void adj_periodic(const FdtdPlan* p, T* lam, bool is_H, hipStream_t st) {
+  for (auto it = p->stages.rbegin(); it != p->stages.rend(); ++it) {
+    const int axis = it->index;
+    const int n_axis = axis == 0 ? p->d.nx : (axis == 1 ? p->d.ny : p->d.nz);
+    if (n_axis <= 1) continue;  // the forward copy is the identity
+    const int64_t n_plane = p->d.cs / n_axis;
+    hipLaunchKernelGGL(k_adj_periodic<T>, dim3((unsigned)((n_plane + 255) / 256)), dim3(256), 0, st, lam, p->d, axis,
+                       is_H ? 0 : 1);
   }
 }
 
@@ -478,6 +505,9 @@ AdjArgs<T> adj_args(const FdtdAdjoint* a) {
   r.inv_mu = reinterpret_cast<const T*>(p->inv_mu);
   r.gcoef = reinterpret_cast<const T*>(a->gcoef);
   r.dcoef = reinterpret_cast<const T*>(a->dcoef);
+  r.objs = p->d_objs;
+  r.nobjs = (int32_t)p->objs.size();
+  r.ybuf = reinterpret_cast<T*>(a->ybuf);
   r.db = a->db;
   r.pb = a->pb;
   r.grad = a->grad;
@@ -532,16 +562,26 @@ int adjoint_steps(FdtdAdjoint* a, int64_t nsteps, const T* curl, const T* plane,
   const int bz = 1 << args.bz_shift, by = args.by;
   const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
   const dim3 grid(tiles * (unsigned)((p->d.nx + args.cx - 1) / args.cx)), block(bz * by);
+  const dim3 ygrid((unsigned)((p->d.cs + 255) / 256));
   for (int64_t s = nsteps - 1; s >= 0; --s) {
+    // each half step transposed in reverse: hard sources, then the periodic
+    // copies (reverse registration order), then the main update
     args.plane = plane + s * 3 * np;
     args.side = adj_sides<T>(a, true, a->parH);
     adj_mask_hard_sources<T>(p, args.lamH, true, st);
+    adj_periodic<T>(p, args.lamH, true, st);
     hipLaunchKernelGGL(k_adj_H<T>, grid, block, 0, st, args);
     a->parH ^= 1;
     args.curl = curl + s * 3 * nd;
     args.side = adj_sides<T>(a, false, a->parE);
     adj_mask_hard_sources<T>(p, args.lamE, false, st);
-    hipLaunchKernelGGL(k_adj_E<T>, grid, block, 0, st, args);
+    adj_periodic<T>(p, args.lamE, false, st);
+    if (args.ybuf) {
+      hipLaunchKernelGGL(k_adj_Y<T>, ygrid, dim3(256), 0, st, args);
+      hipLaunchKernelGGL((k_adj_E<T, true>), grid, block, 0, st, args);
+    } else {
+      hipLaunchKernelGGL((k_adj_E<T, false>), grid, block, 0, st, args);
+    }
     a->parE ^= 1;
   }
   FDTD_HIP(hipGetLastError());
@@ -1262,6 +1302,17 @@ int fdtd_adjoint_create(FdtdPlan* p, const void* gcoef, const void* dcoef, void*
       return fail(-3, std::string("adjoint face table: ") + hipGetErrorString(e));
     }
   }
+  for (const ObjHost& o : p->objs) {
+    if (o.kind != FDTD_OBJ_ANISOTROPIC) continue;
+    const size_t bytes = (size_t)3 * p->d.cs * (p->dtype == FDTD_F64 ? 8 : 4);
+    if (hipMalloc(&a->ybuf, bytes) != hipSuccess) {
+      a->ybuf = nullptr;
+      (void)hipFree(a->d_sides);
+      delete a;
+      return fail(-3, "adjoint Y buffer: out of device memory");
+    }
+    break;
+  }
   a->plan = p;
   a->gcoef = gcoef;
   a->dcoef = dcoef;
@@ -1278,10 +1329,9 @@ int fdtd_adjoint_create(FdtdPlan* p, const void* gcoef, const void* dcoef, void*
 
 int fdtd_adjoint_destroy(FdtdAdjoint* a) {
   if (!a) return 0;
-  if (a->d_sides) {
-    (void)hipDeviceSynchronize();
-    (void)hipFree(a->d_sides);
-  }
+  if (a->d_sides || a->ybuf) (void)hipDeviceSynchronize();
+  if (a->d_sides) (void)hipFree(a->d_sides);
+  if (a->ybuf) (void)hipFree(a->ybuf);
   delete a;
   return 0;
 }

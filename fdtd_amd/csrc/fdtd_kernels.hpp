@@ -1064,6 +1064,11 @@ struct AdjArgs {
   Box pb;                // objective box: lamE += 2 E
   const T* plane;        // (3, cells of pb) recorded for this step
   int32_t bz_shift, cx, by;  // launch shape, as for the main kernels
+  // anisotropic objects: the plan's object table and a (3, N) buffer of
+  // Y = dJ/d(curl H) of the E half step, written by k_adj_Y
+  const ObjDev* objs;
+  int32_t nobjs;
+  T* ybuf;
 };
 
 // Slab layout constants of a face normal to AX: the slab is the grid with
@@ -1281,7 +1286,68 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
 // curl_H^T gathers forward neighbours with the masks of curl_H
 // (grid.py:65-73): (curl_H^T Y)_x = ([k>=1] Y_y - [k<=nz-2] Y_y(k+1))
 // - ([j>=1] Y_z - [j<=ny-2] Y_z(j+1)), and cyclically.
+// Y = dJ/d(curl H) of the E half step at every cell when the plan has
+// anisotropic objects: gcoef lamE (the diagonal part composed by the host)
+// plus, inside each anisotropic object, the transpose of its update
+// E += S (inv @ curl) (objects.py:254-269): S inv^T lamE. Overlapping objects
+// add their terms in the forward step, so their transposes add here too.
 template <typename T>
+__global__ void __launch_bounds__(256) k_adj_Y(AdjArgs<T> a) {
+  const Dims d = a.d;
+  const int64_t cs = d.cs;
+  const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (cell >= cs) return;
+  const T l0 = a.lamE[cell], l1 = a.lamE[cs + cell], l2 = a.lamE[2 * cs + cell];
+  T y0 = a.gcoef[cell] * l0, y1 = a.gcoef[cs + cell] * l1, y2 = a.gcoef[2 * cs + cell] * l2;
+  const int k = (int)(cell % d.nz), j = (int)((cell / d.nz) % d.ny), i = (int)(cell / ((int64_t)d.ny * d.nz));
+  for (int o = 0; o < a.nobjs; ++o) {
+    const ObjDev& ob = a.objs[o];
+    if (ob.kind != FDTD_OBJ_ANISOTROPIC) continue;
+    const Box b{ob.x0, ob.x1, ob.y0, ob.y1, ob.z0, ob.z1};
+    const int64_t lo = box_index(b, i, j, k);
+    if (lo < 0) continue;
+    const gptr<const T> m = gp(reinterpret_cast<const T*>(ob.inv) + lo * 9);
+    y0 = y0 + a.sc * ((m[0] * l0 + m[3] * l1) + m[6] * l2);
+    y1 = y1 + a.sc * ((m[1] * l0 + m[4] * l1) + m[7] * l2);
+    y2 = y2 + a.sc * ((m[2] * l0 + m[5] * l1) + m[8] * l2);
+  }
+  a.ybuf[cell] = y0;
+  a.ybuf[cs + cell] = y1;
+  a.ybuf[2 * cs + cell] = y2;
+}
+
+// Transpose of a periodic plane copy F[dst] = F[src] along `axis`
+// (boundaries.py:184-219; to_low: dst is plane 0, src plane n-1, as for E;
+// else the reverse, as for H): lam[src] += lam[dst], lam[dst] = 0.
+template <typename T>
+__global__ void __launch_bounds__(256) k_adj_periodic(T* lam, Dims d, int axis, int to_low) {
+  const int64_t p = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  int64_t lo_cell, hi_cell;
+  if (axis == 0) {
+    const int64_t n_plane = (int64_t)d.ny * d.nz;
+    if (p >= n_plane) return;
+    lo_cell = p;
+    hi_cell = (int64_t)(d.nx - 1) * n_plane + p;
+  } else if (axis == 1) {
+    if (p >= (int64_t)d.nx * d.nz) return;
+    const int64_t i = p / d.nz, k = p % d.nz;
+    lo_cell = (i * d.ny) * d.nz + k;
+    hi_cell = (i * d.ny + d.ny - 1) * d.nz + k;
+  } else {
+    if (p >= (int64_t)d.nx * d.ny) return;
+    lo_cell = p * d.nz;
+    hi_cell = p * d.nz + d.nz - 1;
+  }
+  const int64_t dst = to_low ? lo_cell : hi_cell, src = to_low ? hi_cell : lo_cell;
+#pragma unroll
+  for (int c = 0; c < 3; ++c) {
+    lam[c * d.cs + src] = lam[c * d.cs + src] + lam[c * d.cs + dst];
+    lam[c * d.cs + dst] = T(0);
+  }
+}
+
+// YB: Y read from ybuf (k_adj_Y ran first) instead of gcoef lamE.
+template <typename T, bool YB>
 __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   const Dims d = a.d;
   const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
@@ -1296,23 +1362,24 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   const int64_t zp = k < d.nz - 1 ? 1 : 0, yp = j < d.ny - 1 ? sy : 0;
   const T zero = T(0), sc = a.sc;
   auto L = [&](int c, int64_t x) { return a.lamE[c * cs + x]; };
-  auto G = [&](int c, int64_t x) { return a.gcoef[c * cs + x]; };
+  // Y of component c at cell x whose lamE is l
+  auto Yc = [&](int c, int64_t x, T l) { return YB ? a.ybuf[c * cs + x] : a.gcoef[c * cs + x] * l; };
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   T yz_next = zero, yy_next = zero;  // Y at plane i+1
   T lz_next = zero, ly_next = zero;  // lamE at plane i+1
   if (i1 < d.nx) {
     lz_next = L(2, cell + sx);
     ly_next = L(1, cell + sx);
-    yz_next = G(2, cell + sx) * lz_next;
-    yy_next = G(1, cell + sx) * ly_next;
+    yz_next = Yc(2, cell + sx, lz_next);
+    yy_next = Yc(1, cell + sx, ly_next);
   }
   for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
     const T le0 = L(0, cell), le1 = L(1, cell), le2 = L(2, cell);
     const T lk0 = L(0, cell + zp), lk1 = L(1, cell + zp);
     const T lj0 = L(0, cell + yp), lj2 = L(2, cell + yp);
-    const T y0 = G(0, cell) * le0, y1 = G(1, cell) * le1, y2 = G(2, cell) * le2;
-    const T yy_kp = G(1, cell + zp) * lk1, yx_kp = G(0, cell + zp) * lk0;
-    const T yz_jp = G(2, cell + yp) * lj2, yx_jp = G(0, cell + yp) * lj0;
+    const T y0 = Yc(0, cell, le0), y1 = Yc(1, cell, le1), y2 = Yc(2, cell, le2);
+    const T yy_kp = Yc(1, cell + zp, lk1), yx_kp = Yc(0, cell + zp, lk0);
+    const T yz_jp = Yc(2, cell + yp, lj2), yx_jp = Yc(0, cell + yp, lj0);
     const T l0 = a.lamH[cell], l1 = a.lamH[cs + cell], l2 = a.lamH[2 * cs + cell];
     const bool km = k >= 1, kp = k <= d.nz - 2, jm = j >= 1, jp = j <= d.ny - 2, im = i >= 1, ip = i <= d.nx - 2;
     const T t0 = ((km ? y1 : zero) - (kp ? yy_kp : zero)) - ((jm ? y2 : zero) - (jp ? yz_jp : zero));

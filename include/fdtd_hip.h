@@ -240,8 +240,10 @@ int fdtd_batch_run_static(int n, FdtdPlan* const* plans, int block_threads, int 
  *                      inside the objective box;
  *   fdtd_adjoint_run   the transposed steps in reverse order over such a
  *                      record, accumulating dJ/d(inverse permittivity).
- * Plans with periodic boundaries, anisotropic objects or FDTD_F32C storage
- * are rejected. */
+ * Periodic boundaries (registered after every PML face) and anisotropic
+ * objects (not overlapping an absorbing one) are transposed too. Rejected:
+ * FDTD_F32C and complex storage, a PML face registered after a periodic
+ * boundary, more than one PML face per side, soft sources. */
 typedef struct FdtdAdjoint FdtdAdjoint;
 
 /* Elements of the adjoint CPML state buffer (plan dtype) that
@@ -252,8 +254,10 @@ int fdtd_adjoint_psi_size(const FdtdPlan* plan, int64_t* elements);
  *   gcoef, dcoef  (3, Nx, Ny, Nz): the E update as E' = dcoef E + gcoef
  *                 curl_H(H) + CPML + sources, composed by the caller from the
  *                 grid and the objects (S inv_eps outside objects, S inv_obj
- *                 inside a plain object, absorbing objects scale both);
- *                 dcoef may be NULL (all ones)
+ *                 inside a plain object, absorbing objects scale both;
+ *                 the 3x3 updates of anisotropic objects are not in gcoef:
+ *                 the library adds their transposes from the plan's object
+ *                 arrays); dcoef may be NULL (all ones)
  *   lamE, lamH    (3, Nx, Ny, Nz) adjoint fields, zeroed by the caller
  *   psi_adj       fdtd_adjoint_psi_size elements, zeroed by the caller
  *   grad          float64 (3, cells of the design box), z fastest, zeroed by

@@ -17,6 +17,8 @@ the device library gets. Prints one JSON line per case: {"case", "ok",
   random_complex:<seed>  the same on the complex-field path
   compensated:<name>  a golden scenario in hip.float32c: relative L2 error of
                       the fields against the fixture <= 1e-7
+  adjoint:<name>      the gradient of a gradient scenario (record mode)
+                      against the reference-autograd fixture, 1e-10
   batch:<seed>        grids of uneven shapes stepped by fdtd_batch_run
                       (direct launches, graph replays and a persistent
                       launch) and by fdtd_batch_run_static, bit for bit
@@ -77,6 +79,13 @@ def load():
             grid_mod.bd = real_bd
 
     grid_mod.Grid._ensure_plan = ensure_plan
+    # the gradient's plan and stream
+    import types
+
+    import fdtd_amd.adjoint as adj_mod
+
+    adj_mod._require_device = lambda t: None
+    torch.cuda.current_stream = lambda *a, **k: types.SimpleNamespace(cuda_stream=None)
     return L
 
 
@@ -161,6 +170,29 @@ def case_compensated(name):
     return errs
 
 
+def case_adjoint(name):
+    """The gradient (record mode) against the reference-autograd fixture
+    adjoint_<name>.npz: J and dJ/d permittivity within 1e-10."""
+    import adjoint_scenarios as A
+    from fdtd_amd.adjoint import IntensityGradient
+
+    fdtd.set_backend("hip.float64")
+    sc = {"g5": A.G5, "pa": A.PA}[name]
+    g = np.load(os.path.join(ROOT, "tests", "golden", f"adjoint_{name}.npz"))
+    perm = A.design_permittivity(sc)
+    grid, design = A.build(fdtd, sc, torch.from_numpy(perm))
+    prob = IntensityGradient(grid, design, sc["plane"], sc["steps"], checkpoint="record")
+    J, grad = prob.gradient(torch.from_numpy(perm))
+    grad = grad.numpy()
+    errs = []
+    if abs(J - float(g["J"])) > 1e-10 * float(g["J"]):
+        errs.append(f"J {J!r} against {float(g['J'])!r}")
+    r = np.linalg.norm(grad - g["grad"]) / np.linalg.norm(g["grad"])
+    if not r <= 1e-10:
+        errs.append(f"gradient: relative L2 {r:.3e} against the reference autograd")
+    return errs
+
+
 def _batch_grids(seed):
     rs = np.random.RandomState(int(seed))
     grids = []
@@ -224,7 +256,7 @@ def case_batch(seed):
 
 
 CASES = {"fixture": case_fixture, "random": case_random, "random_complex": case_random_complex,
-         "compensated": case_compensated, "batch": case_batch}
+         "compensated": case_compensated, "batch": case_batch, "adjoint": case_adjoint}
 
 
 def main(argv):

@@ -66,6 +66,51 @@ TINY_ABSORBER = dict(TINY, name="tiny_absorber",
 TINY_PLANE = dict(TINY, name="tiny_plane",
                   plane_source=((10, slice(4, 14), slice(2, 12)), dict(period=12, amplitude=0.3, polarization="y")))
 
+# Periodic in z (no z faces: the PML faces on x and y are registered first,
+# then the PeriodicBoundary) with an AnisotropicObject between the design
+# region and the objective plane whose inverse permittivity is overwritten by
+# a seeded per-cell SPD inverse (eigenvalues 1.5-4); the reference's torch
+# autograd gives the fixture gradient at this size
+PA = dict(
+    name="pa",
+    shape=(36, 32, 12),
+    spacing=50e-9,
+    courant=3 ** -0.5,
+    pml=6,
+    steps=150,
+    periodic_z=True,
+    source=(9, slice(10, 22), 6),
+    design=(slice(13, 21), slice(10, 22), slice(2, 10)),
+    plane=(28, slice(12, 20), slice(3, 9)),
+    aniso=((slice(22, 25), slice(8, 24), slice(0, 12)), 13),
+    seed=9,
+)
+
+# the same features on TINY (central differences over every design voxel)
+TINY_PA = dict(TINY, name="tiny_pa", periodic_z=True, aniso=((slice(13, 15), slice(4, 14), slice(0, 14)), 17))
+
+
+def aniso_inverse(sc):
+    """The overwritten inverse permittivity of the anisotropic object:
+    (nx, ny, nz, 3, 3), per cell the inverse of a seeded SPD matrix."""
+    from scenarios import _spd_inverse_field
+
+    key, seed = sc["aniso"]
+    shape = tuple(k.stop - k.start for k in key)
+    return _spd_inverse_field(np.random.RandomState(seed), shape, 1.5, 4.0)
+
+
+def pml_keys(sc):
+    """The PML faces, in registration order: x and y, and z unless z is
+    periodic."""
+    t = sc["pml"]
+    s = slice
+    keys = [(s(0, t), s(None), s(None)), (s(-t, None), s(None), s(None)),
+            (s(None), s(0, t), s(None)), (s(None), s(-t, None), s(None))]
+    if not sc.get("periodic_z"):
+        keys += [(s(None), s(None), s(0, t)), (s(None), s(None), s(-t, None))]
+    return keys
+
 
 def design_shape(sc):
     return tuple(s.stop - s.start for s in sc["design"])
@@ -81,13 +126,11 @@ def build(mod, sc, permittivity):
     """Grid + registered components in a fixed order; returns (grid, design)."""
     grid = mod.Grid(sc["shape"], grid_spacing=sc["spacing"], permittivity=1.0, permeability=1.0,
                     courant_number=sc["courant"])
-    t = sc["pml"]
-    grid[0:t, :, :] = mod.PML(name="pml_xlow")
-    grid[-t:, :, :] = mod.PML(name="pml_xhigh")
-    grid[:, 0:t, :] = mod.PML(name="pml_ylow")
-    grid[:, -t:, :] = mod.PML(name="pml_yhigh")
-    grid[:, :, 0:t] = mod.PML(name="pml_zlow")
-    grid[:, :, -t:] = mod.PML(name="pml_zhigh")
+    names = ["pml_xlow", "pml_xhigh", "pml_ylow", "pml_yhigh", "pml_zlow", "pml_zhigh"]
+    for key, name in zip(pml_keys(sc), names):
+        grid[key] = mod.PML(name=name)
+    if sc.get("periodic_z"):
+        grid[:, :, 0] = mod.PeriodicBoundary(name="zbounds")
     grid[sc["source"]] = mod.LineSource(period=1550e-9 / C0, pulse=True, name="source")
     design = mod.Object(permittivity, name="design")
     grid[sc["design"]] = design
@@ -97,6 +140,12 @@ def build(mod, sc, permittivity):
     if sc.get("plane_source"):
         key, kw = sc["plane_source"]
         grid[key] = mod.PlaneSource(name="plane_source", **kw)
+    if sc.get("aniso"):
+        import torch
+
+        obj = mod.AnisotropicObject(permittivity=2.0, name="aniso")
+        grid[sc["aniso"][0]] = obj
+        obj.inverse_permittivity = torch.from_numpy(aniso_inverse(sc))
     return grid, design
 
 
@@ -111,12 +160,10 @@ def sample_voxels(sc, n, seed=11):
 def build_oracle(oracle_mod, sc, permittivity):
     """The same scenario through the oracle's registration API; returns sim."""
     sim = oracle_mod.Sim(sc["shape"], sc["spacing"], 1.0, 1.0, sc["courant"])
-    t = sc["pml"]
-    s = slice
-    for key in ((s(0, t), s(None), s(None)), (s(-t, None), s(None), s(None)),
-                (s(None), s(0, t), s(None)), (s(None), s(-t, None), s(None)),
-                (s(None), s(None), s(0, t)), (s(None), s(None), s(-t, None))):
+    for key in pml_keys(sc):
         sim.add("PML", key)
+    if sc.get("periodic_z"):
+        sim.add("PeriodicBoundary", (slice(None), slice(None), 0))
     sim.add("LineSource", sc["source"], period=1550e-9 / C0, pulse=True)
     sim.add("Object", sc["design"], permittivity=permittivity)
     if sc.get("absorber"):
@@ -125,6 +172,9 @@ def build_oracle(oracle_mod, sc, permittivity):
     if sc.get("plane_source"):
         key, kw = sc["plane_source"]
         sim.add("PlaneSource", key, **kw)
+    if sc.get("aniso"):
+        obj = sim.add("AnisotropicObject", sc["aniso"][0], permittivity=2.0)
+        obj.inverse_permittivity = aniso_inverse(sc)
     return sim
 
 
