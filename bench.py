@@ -317,7 +317,7 @@ def precision_form(args):
     print(json.dumps(out), flush=True)
 
 
-def adjoint_measure(dtype="float32", checkpoints=("sqrt", "record"), reps=3):
+def adjoint_measure(dtype="float32", checkpoints=("sqrt", "record"), reps=3, steps=None):
     """Gradient cost on the differentiation configuration (128x128x64, 12-cell
     PML, 64x64x16 design Object with seeded eps in [1, 12], pulsed 1550 nm
     line source, J = time-integrated |E|^2 over a 16x16 plane, 2000 steps):
@@ -336,7 +336,7 @@ def adjoint_measure(dtype="float32", checkpoints=("sqrt", "record"), reps=3):
     fdtd.set_backend(f"hip.{dtype}")
     perm = torch.from_numpy(A.design_permittivity(sc))
     grid, design = A.build(fdtd, sc, perm)
-    T = sc["steps"]
+    T = steps or sc["steps"]
 
     def timed(fn):
         best = None
@@ -425,7 +425,7 @@ def batch_grids(fdtd, torch, count=64, seed=0):
     return [batch_grid(fdtd, spec) for spec in batch_specs(count, seed)]
 
 
-def batch_measure(steps=2000, seq_steps=200, count=64):
+def batch_measure(steps=2000, seq_steps=200, count=64, launch=None):
     """Aggregate throughput of the batched step (autotuned launch shape, HIP
     graph replays) on the batching configuration, beside the same grids run
     one after another through the single-grid path."""
@@ -439,8 +439,12 @@ def batch_measure(steps=2000, seq_steps=200, count=64):
     cells = sum(g.Nx * g.Ny * g.Nz for g in grids)
     slab_cells = sum(2 * 10 * (g.Ny * g.Nz + g.Nx * g.Nz + g.Nx * g.Ny) for g in grids)
     step_bytes = 2 * (cells * 12 * 4 + slab_cells * 4 * 4)
-    batch = GridBatch(grids, graph_steps=100)
-    tune = batch.autotune(steps=10)
+    batch = GridBatch(grids, graph_steps=min(100, steps))
+    if launch:  # a fixed launch shape (block threads, x planes, workgroups): no autotuning
+        batch.set_launch(*launch)
+        tune = {}
+    else:
+        tune = batch.autotune(steps=10)
     stream = torch.cuda.current_stream()
 
     def timed(fn):
@@ -453,7 +457,7 @@ def batch_measure(steps=2000, seq_steps=200, count=64):
         e1.synchronize()
         return e0.elapsed_time(e1)
 
-    batch.run(100)  # graph capture and warm-up
+    batch.run(min(100, steps))  # graph capture and warm-up
     ms = timed(lambda: batch.run(steps))
     ms_direct = timed(lambda: batch.run(seq_steps, graph=False)) / seq_steps
     ms_seq = timed(lambda: [g.run(seq_steps, progress_bar=False) for g in grids]) / seq_steps
@@ -478,7 +482,12 @@ def batch_measure(steps=2000, seq_steps=200, count=64):
 
 
 def adjoint_form(args):
-    out = adjoint_measure("float64" if args.adjoint_f64 else "float32")
+    import torch
+
+    copy_gbs = copy_bandwidth(torch, 1 << 28, 2)  # also the scale of byte counters in a profiled run
+    out = adjoint_measure("float64" if args.adjoint_f64 else "float32", reps=args.adjoint_reps,
+                          steps=args.adjoint_steps)
+    out["copy_GBps"] = round(copy_gbs, 1)
     out["higher_is_better"] = False
     print(json.dumps(out), flush=True)
 
@@ -505,6 +514,11 @@ def main():
                     help="grid material as a <=256-entry palette (auto: in the compensated form only)")
     ap.add_argument("--adjoint-f64", action="store_true", help="--form adjoint in float64")
     ap.add_argument("--shape", type=int, nargs=3, default=None, help="grid shape for --form precision")
+    ap.add_argument("--adjoint-steps", type=int, default=None, help="--form adjoint: steps (default: the workload's 2000)")
+    ap.add_argument("--adjoint-reps", type=int, default=3, help="--form adjoint: timed repetitions (best of)")
+    ap.add_argument("--batch-steps", type=int, default=2000, help="--form batch: timed steps")
+    ap.add_argument("--batch-launch", default=None,
+                    help="--form batch: fixed launch shape 'threads,cx,workgroups' instead of autotuning")
     args = ap.parse_args()
     global PALETTE
     PALETTE = args.palette
@@ -517,7 +531,12 @@ def main():
     if args.form == "adjoint":
         return adjoint_form(args)
     if args.form == "batch":
-        out = batch_measure()
+        import torch
+
+        copy_gbs = copy_bandwidth(torch, 1 << 28, 2)  # also the scale of byte counters in a profiled run
+        launch = tuple(int(v) for v in args.batch_launch.split(",")) if args.batch_launch else None
+        out = batch_measure(steps=args.batch_steps, seq_steps=min(200, args.batch_steps), launch=launch)
+        out["copy_GBps"] = round(copy_gbs, 1)
         out["higher_is_better"] = True
         print(json.dumps(out), flush=True)
         return
