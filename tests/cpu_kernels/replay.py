@@ -228,6 +228,21 @@ def _state(grids):
     return [t.clone() for t in out]
 
 
+def _run_batch(mode, launch, grids, steps):
+    """Step `grids` by `steps` through one of the batched entry points."""
+    plans = [g._prepare(steps, steps, steps) for g in grids]
+    if mode == "static":
+        _native.batch_run_static(plans, *launch, 0, steps, None)
+        return []
+    b = _native.Batch(plans)
+    b.set_launch(*launch)
+    if mode == "graph":
+        b.run_graph(0, steps, 4, None)
+    else:
+        b.run(0, steps, None)
+    return []
+
+
 def case_batch(seed):
     fdtd.set_backend("hip.float64")
     steps = 9
@@ -239,16 +254,7 @@ def case_batch(seed):
     for mode, launch in (("direct", (256, 1, 0)), ("graph", (256, 2, 0)), ("persistent", (512, 1, 3)),
                          ("static", (256, 2, 0))):
         grids = _batch_grids(seed)
-        plans = [g._prepare(steps, steps, steps) for g in grids]
-        if mode == "static":
-            _native.batch_run_static(plans, *launch, 0, steps, None)
-        else:
-            b = _native.Batch(plans)
-            b.set_launch(*launch)
-            if mode == "graph":
-                b.run_graph(0, steps, 4, None)
-            else:
-                b.run(0, steps, None)
+        errs += _run_batch(mode, launch, grids, steps)
         for i, (x, y) in enumerate(zip(_state(grids), want)):
             if not torch.equal(x, y):
                 errs.append(f"{mode} {launch}: state tensor {i} differs")
