@@ -45,6 +45,7 @@ _SIGNATURES = {
     "fdtd_half_step_E": ([_vp, _c_i64, _vp], _c_int),
     "fdtd_half_step_H": ([_vp, _c_i64, _vp], _c_int),
     "fdtd_run": ([_vp, _c_i64, _c_i64, _vp], _c_int),
+    "fdtd_check_finite": ([_vp, _vp, _vp], _c_int),
     "fdtd_bench_main_kernel": ([_vp, _c_i64, _vp, ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float)],
                                _c_int),
     "fdtd_bench_copy": ([_vp, _vp, _c_i64, _c_i64, _vp, ctypes.POINTER(ctypes.c_float)], _c_int),
@@ -219,6 +220,11 @@ class Plan:
 
     def run(self, q0, nsteps, stream):
         check(self._L.fdtd_run(self._h, q0, nsteps, stream))
+
+    def check_finite(self, flag, stream):
+        """OR 1 into the device int32 tensor `flag` if an E or H value is
+        non-finite (queued on `stream`)."""
+        check(self._L.fdtd_check_finite(self._h, flag.data_ptr(), stream))
 
     def bench_main_kernel(self, reps, stream):
         """Average milliseconds of the E and H main kernels over `reps` steps."""

@@ -1262,6 +1262,36 @@ int fdtd_run(FdtdPlan* p, int64_t q0, int64_t nsteps, void* stream) {
   return 0;
 }
 
+int fdtd_check_finite(const FdtdPlan* p, int32_t* flag, void* stream) {
+  if (!p) return fail(-1, "null plan");
+  if (!flag) return fail(-1, "null flag");
+  if (int rc = select_device(p)) return rc;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const int64_t cs = p->d.cs;
+  const dim3 grid(1024), block(256);
+  for (void* f : {p->E, p->H}) {
+    switch (p->dtype) {
+      case FDTD_F64:
+        hipLaunchKernelGGL(k_nonfinite<double>, grid, block, 0, st, (const double*)f, 3 * cs, 0, flag);
+        break;
+      case FDTD_C128:
+        hipLaunchKernelGGL(k_nonfinite<double>, grid, block, 0, st, (const double*)f, 6 * cs, 0, flag);
+        break;
+      case FDTD_F32:
+        hipLaunchKernelGGL(k_nonfinite<float>, grid, block, 0, st, (const float*)f, 3 * cs, 0, flag);
+        break;
+      case FDTD_C64:
+        hipLaunchKernelGGL(k_nonfinite<float>, grid, block, 0, st, (const float*)f, 6 * cs, 0, flag);
+        break;
+      default:  // FDTD_F32C: (N, 4) records, the 4th float is the compensation word
+        hipLaunchKernelGGL(k_nonfinite<float>, grid, block, 0, st, (const float*)f, 4 * cs, 4, flag);
+        break;
+    }
+  }
+  FDTD_HIP(hipGetLastError());
+  return 0;
+}
+
 int fdtd_adjoint_psi_size(const FdtdPlan* p, int64_t* elements) {
   if (!p || !elements) return fail(-1, "null argument");
   int64_t n = 0;

@@ -1010,6 +1010,23 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
   }
 }
 
+// Non-finite check: flag[0] becomes nonzero when one of the n values of v
+// (stride `skip`: with skip = 4 every 4th value is not a field value but a
+// compensation word and is ignored) is a NaN or an infinity. The reference
+// has no such check (its only guard is the Courant limit, grid.py:123-127);
+// it lets a caller find a diverged run without copying the fields back.
+template <typename S>
+__global__ void __launch_bounds__(256) k_nonfinite(const S* __restrict__ v, int64_t n, int skip, int* flag) {
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) {
+    if (skip && i % skip == skip - 1) continue;
+    const S x = v[i];
+    bad |= !(x - x == S(0));  // NaN - NaN and inf - inf are NaN
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
 // Plain device copy of n 16-byte words: the bandwidth reference for the
 // step kernels (a stream that reads and writes every byte once).
 __global__ void __launch_bounds__(256) k_copy(const uint4* __restrict__ src, uint4* __restrict__ dst, int64_t n) {

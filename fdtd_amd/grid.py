@@ -459,6 +459,17 @@ class Grid:
         if bar is not None:
             bar.close()
 
+    def has_nonfinite(self) -> bool:
+        """True when a value of E or H is a NaN or an infinity: one check on
+        the device and one 4-byte read-back (no counterpart in the reference,
+        whose only guard against divergence is the Courant limit at
+        construction, grid.py:123-127 -- a courant_number raised afterwards,
+        or a NaN written into the fields, goes unnoticed there)."""
+        plan = self._ensure_plan()
+        flag = torch.zeros(1, dtype=torch.int32, device=self._E.device)
+        plan.check_finite(flag, self._stream())
+        return bool(flag.item())
+
     def step(self):
         """E half step, H half step, then advance the time index (grid.py:267-273)."""
         self.update_E()

@@ -174,6 +174,13 @@ int fdtd_half_step_H(FdtdPlan* plan, int64_t q, void* stream);
  * E half step then H half step at q, then q+1. */
 int fdtd_run(FdtdPlan* plan, int64_t q0, int64_t nsteps, void* stream);
 
+/* No counterpart in the reference, whose only guard against divergence is
+ * the Courant limit at construction (fdtd/grid.py:123-127): launch on
+ * `stream` a check of every E and H value of the plan and OR 1 into the
+ * device int *flag (zeroed by the caller) if one is a NaN or an infinity.
+ * The plan need not be finalized. */
+int fdtd_check_finite(const FdtdPlan* plan, int32_t* flag, void* stream);
+
 /* ---- Batches of grids ----------------------------------------------------
  * No counterpart in the reference (one Grid steps at a time there,
  * fdtd/grid.py:250-273): many independent grids, each a finalized plan with

@@ -95,6 +95,13 @@ inline float __uint_as_float(unsigned v) {
   return f;
 }
 
+// work-items run one at a time on one host thread
+inline int atomicOr(int* p, int v) {
+  const int old = *p;
+  *p = old | v;
+  return old;
+}
+
 struct uint4 {
   unsigned x, y, z, w;
 };

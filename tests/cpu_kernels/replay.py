@@ -17,6 +17,8 @@ the device library gets. Prints one JSON line per case: {"case", "ok",
   random_complex:<seed>  the same on the complex-field path
   compensated:<name>  a golden scenario in hip.float32c: relative L2 error of
                       the fields against the fixture <= 1e-7
+  nonfinite:<name>    Grid.has_nonfinite after a golden scenario, and with a
+                      NaN / an infinity written into E / H
   adjoint:<name>      the gradient of a gradient scenario (record mode)
                       against the reference-autograd fixture, 1e-10
   batch:<seed>        grids of uneven shapes stepped by fdtd_batch_run
@@ -193,6 +195,30 @@ def case_adjoint(name):
     return errs
 
 
+def case_nonfinite(name):
+    """Grid.has_nonfinite on a golden scenario: False after its run, True
+    after a NaN or an infinity is written into one E or H value."""
+    errs = []
+    sc = scenarios.get_scenario(name)
+    complex_ = name.startswith("complex")
+    for backend in ("hip.float64", "hip.float32" if complex_ else "hip.float32c"):
+        fdtd.set_backend(backend)
+        grid, _ = scenarios.build_fdtd(fdtd, sc, to_array=torch.from_numpy)
+        scenarios.run_program(grid, sc)
+        if grid.has_nonfinite():
+            errs.append(f"{backend}: finite run flagged")
+        for field, bad in (("E", float("nan")), ("H", float("inf"))):
+            f = getattr(grid, field)
+            keep = f[1, 2, 3, 1].clone()
+            f[1, 2, 3, 1] = bad
+            if not grid.has_nonfinite():
+                errs.append(f"{backend}: {bad} in {field} not flagged")
+            f[1, 2, 3, 1] = keep
+        if grid.has_nonfinite():
+            errs.append(f"{backend}: flag stuck after restoring the value")
+    return errs
+
+
 def _batch_grids(seed):
     rs = np.random.RandomState(int(seed))
     grids = []
@@ -262,7 +288,8 @@ def case_batch(seed):
 
 
 CASES = {"fixture": case_fixture, "random": case_random, "random_complex": case_random_complex,
-         "compensated": case_compensated, "batch": case_batch, "adjoint": case_adjoint}
+         "compensated": case_compensated, "batch": case_batch, "adjoint": case_adjoint,
+         "nonfinite": case_nonfinite}
 
 
 def main(argv):

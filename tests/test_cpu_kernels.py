@@ -9,7 +9,8 @@ oracle; the compensated form stays within its bound; batched runs (direct,
 graph replay, persistent, and the static-table entry point) equal the grids
 stepped alone; the transposed steps give the reference-autograd gradients of
 both gradient fixtures (all PML faces; periodic z with an anisotropic
-object).
+object); the non-finite check flags a NaN or an infinity in E or H and
+nothing else.
 Both tile shapes of the main kernels run: the default (flat tiles for these
 grids, whose nz is not a multiple of 64) and row tiles forced with
 FDTD_HIP_FLAT=0. Any out-of-bounds access aborts the child process with an
@@ -33,7 +34,8 @@ def _cases():
     out = [f"fixture:{sc['name']}" for sc in scenarios.all_scenarios()]
     out += [f"random:{s}" for s in range(6)]
     out += [f"random_complex:{s}" for s in range(6)]
-    out += ["compensated:g2", "compensated:periodic_x_first", "batch:1", "batch:2", "adjoint:g5", "adjoint:pa"]
+    out += ["compensated:g2", "compensated:periodic_x_first", "batch:1", "batch:2", "adjoint:g5", "adjoint:pa",
+            "nonfinite:g2", "nonfinite:complex_object"]
     return out
 
 

@@ -446,3 +446,29 @@ def test_complex_random_scenarios_match_oracle(seed):
     if sc["shape"][0] > 1 and sc["shape"][1] > 1:
         ref = np.sum(sim.E ** 2 + sim.H ** 2, -1)[:, :, z]
         np.testing.assert_allclose(_np(grid.grid_energy(z=z)), ref, rtol=1e-15, atol=1e-15 * np.abs(ref).max())
+
+
+@pytest.mark.parametrize("backend", ["hip.float64", "hip.float32", "hip.float32c"])
+def test_nonfinite_check_flags_a_diverged_run(backend):
+    """Grid.has_nonfinite: False after a normal run; True after a NaN is
+    written into E, and after a run whose courant_number was raised above
+    the 3D limit after construction (the reference checks the limit only at
+    construction, grid.py:123-127, and then diverges silently)."""
+    fdtd.set_backend(backend)
+    rs = np.random.RandomState(3)
+    grid = fdtd.Grid((24, 20, 16), grid_spacing=50e-9, courant_number=0.5)
+    grid[0:4, :, :] = fdtd.PML()
+    grid[12, 10, 8] = fdtd.PointSource(period=10)
+    grid.run(40, progress_bar=False)
+    assert not grid.has_nonfinite()
+    keep = grid.E[5, 6, 7, 2].clone()
+    grid.E[5, 6, 7, 2] = float("nan")
+    assert grid.has_nonfinite()
+    grid.E[5, 6, 7, 2] = keep
+    assert not grid.has_nonfinite()
+    grid.E = torch.from_numpy(rs.uniform(-1, 1, size=(24, 20, 16, 3)))
+    grid.courant_number = 1.5
+    grid.run(1500, progress_bar=False)
+    _say(f"[gpu] {backend}: courant 1.5 after construction, 1500 steps: max |E| "
+         f"{float(grid.E.abs().max()) if not grid.has_nonfinite() else float('nan')}")
+    assert grid.has_nonfinite()
