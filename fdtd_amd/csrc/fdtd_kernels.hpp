@@ -186,6 +186,8 @@ template <typename S, typename R_>
 struct PlainField {
   using R = R_;
   using Sample = S;
+  static constexpr bool kRecords = false;  // the marching kernels' record path (CompField) is off
+  struct Rec {};
   S* p;
   int64_t cs;
   template <int C>
@@ -249,6 +251,13 @@ typedef float f32x4 __attribute__((vector_size(16)));
 struct CompField {
   using R = double;
   using Sample = float;
+  // The marching kernels keep the previous plane's record as loaded and
+  // decode it where it is used: a decoded carry is computed right after its
+  // load, ahead of the branch that guards a cell's store, so the wave would
+  // wait for that load before issuing the plane's others (two memory round
+  // trips per plane instead of one).
+  static constexpr bool kRecords = true;
+  using Rec = f32x4;
   float* rec;  // (N, 4): x, y, z rounded to float, then the compensation word
   int64_t cs;
   __device__ __forceinline__ int comp_of(int64_t i) const { return i >= 2 * cs ? 2 : (i >= cs ? 1 : 0); }
@@ -280,17 +289,21 @@ struct CompField {
   }
   // detectors sample the value rounded to float
   __device__ __forceinline__ float sample(int64_t i) const { return (float)get(i); }
+  __device__ __forceinline__ Rec rec3(int64_t cell) const { return record<false>(cell); }
   template <int C>
-  __device__ __forceinline__ double get_c(int64_t cell) const {
-    const f32x4 r = record<false>(cell);
+  static __device__ __forceinline__ double decode_c(const f32x4& r) {
     return comp_decode(r[C], (__float_as_uint(r[3]) >> (8 * C)) & 0xffu);
   }
-  template <bool NT>
-  __device__ __forceinline__ void get3_s(int64_t cell, double v[3]) const {
-    const f32x4 r = record<NT>(cell);
+  static __device__ __forceinline__ void decode3(const f32x4& r, double v[3]) {
     const uint32_t w = __float_as_uint(r[3]);
 #pragma unroll
     for (int c = 0; c < 3; ++c) v[c] = comp_decode(r[c], (w >> (8 * c)) & 0xffu);
+  }
+  template <int C>
+  __device__ __forceinline__ double get_c(int64_t cell) const { return decode_c<C>(record<false>(cell)); }
+  template <bool NT>
+  __device__ __forceinline__ void get3_s(int64_t cell, double v[3]) const {
+    decode3(record<NT>(cell), v);
   }
   template <bool NT>
   __device__ __forceinline__ void put3_s(int64_t cell, const double v[3]) const {
@@ -626,18 +639,35 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
   const T zero = T(0);
   int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
   T hz_prev = zero, hy_prev = zero;
+  typename FS::Rec hrec{};  // records: H of plane i-1 as loaded
   if (i0 > 0) {
-    hz_prev = a.G.template get_c<2>(cell - sx);
-    hy_prev = a.G.template get_c<1>(cell - sx);
+    if constexpr (FS::kRecords) {
+      hrec = a.G.rec3(cell - sx);
+    } else {
+      hz_prev = a.G.template get_c<2>(cell - sx);
+      hy_prev = a.G.template get_c<1>(cell - sx);
+    }
   }
   typename MS::Key key = a.M.template key<NT>(cell);
   for (int i = i0; i < i1; ++i, cell += sx) {
     T hv[3], m[3], e[3];
     const typename MS::Key kc_ = key;
     if (i + 1 < i1) key = a.M.template key<NT>(cell + sx);  // the next plane's, loaded with this one
-    a.G.template get3_s<false>(cell, hv);
-    const T hy_km = a.G.template get_c<1>(cell - zm), hx_km = a.G.template get_c<0>(cell - zm);
-    const T hz_jm = a.G.template get_c<2>(cell - ym), hx_jm = a.G.template get_c<0>(cell - ym);
+    T hy_km, hx_km, hz_jm, hx_jm;
+    if constexpr (FS::kRecords) {
+      const typename FS::Rec hr = a.G.rec3(cell);
+      FS::decode3(hr, hv);
+      const f32x4 zr = a.G.rec3(cell - zm), yr = a.G.rec3(cell - ym);
+      hx_km = FS::template decode_c<0>(zr), hy_km = FS::template decode_c<1>(zr);
+      hx_jm = FS::template decode_c<0>(yr), hz_jm = FS::template decode_c<2>(yr);
+      hz_prev = FS::template decode_c<2>(hrec);
+      hy_prev = FS::template decode_c<1>(hrec);
+      hrec = hr;
+    } else {
+      a.G.template get3_s<false>(cell, hv);
+      hy_km = a.G.template get_c<1>(cell - zm), hx_km = a.G.template get_c<0>(cell - zm);
+      hz_jm = a.G.template get_c<2>(cell - ym), hx_jm = a.G.template get_c<0>(cell - ym);
+    }
     a.M.template get3<NT>(kc_, cell, m);
     a.F.template get3_s<NT>(cell, e);
     const T hx = hv[0], hy = hv[1], hz = hv[2];
@@ -674,8 +704,10 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
         if (a.pt_cell[s] == cell) e[2] = e[2] + (T)gp(a.wave)[s * a.wave_cap + q];
       a.F.template put3_s<NT>(cell, e);
     }
-    hz_prev = hz;
-    hy_prev = hy;
+    if constexpr (!FS::kRecords) {
+      hz_prev = hz;
+      hy_prev = hy;
+    }
   }
 }
 
@@ -728,18 +760,35 @@ __device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__
   const T zero = T(0);
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   T ez_next = zero, ey_next = zero;  // E at plane i+1
+  typename FS::Rec erec{};          // records: E of plane i+1 as loaded
   if (i1 < d.nx) {
-    ez_next = a.G.template get_c<2>(cell + sx);
-    ey_next = a.G.template get_c<1>(cell + sx);
+    if constexpr (FS::kRecords) {
+      erec = a.G.rec3(cell + sx);
+    } else {
+      ez_next = a.G.template get_c<2>(cell + sx);
+      ey_next = a.G.template get_c<1>(cell + sx);
+    }
   }
   typename MS::Key key = a.M.template key<NT>(cell);
   for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
     T ev[3], m[3], h[3];
     const typename MS::Key kc_ = key;
     if (i > i0) key = a.M.template key<NT>(cell - sx);  // the next plane's (marching -x)
-    a.G.template get3_s<false>(cell, ev);
-    const T ey_kp = a.G.template get_c<1>(cell + zp), ex_kp = a.G.template get_c<0>(cell + zp);
-    const T ez_jp = a.G.template get_c<2>(cell + yp), ex_jp = a.G.template get_c<0>(cell + yp);
+    T ey_kp, ex_kp, ez_jp, ex_jp;
+    if constexpr (FS::kRecords) {
+      const typename FS::Rec er = a.G.rec3(cell);
+      FS::decode3(er, ev);
+      const f32x4 zr = a.G.rec3(cell + zp), yr = a.G.rec3(cell + yp);
+      ex_kp = FS::template decode_c<0>(zr), ey_kp = FS::template decode_c<1>(zr);
+      ex_jp = FS::template decode_c<0>(yr), ez_jp = FS::template decode_c<2>(yr);
+      ez_next = FS::template decode_c<2>(erec);
+      ey_next = FS::template decode_c<1>(erec);
+      erec = er;
+    } else {
+      a.G.template get3_s<false>(cell, ev);
+      ey_kp = a.G.template get_c<1>(cell + zp), ex_kp = a.G.template get_c<0>(cell + zp);
+      ez_jp = a.G.template get_c<2>(cell + yp), ex_jp = a.G.template get_c<0>(cell + yp);
+    }
     a.M.template get3<NT>(kc_, cell, m);
     a.F.template get3_s<NT>(cell, h);
     const T ex = ev[0], ey = ev[1], ez = ev[2];
@@ -766,8 +815,10 @@ __device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__
       if (a.nfaces) pml_cell<false>(a.faces, a.nfaces, d, i, j, k, df, kc, h);
       a.F.template put3_s<NT>(cell, h);
     }
-    ez_next = ez;
-    ey_next = ey;
+    if constexpr (!FS::kRecords) {
+      ez_next = ez;
+      ey_next = ey;
+    }
   }
 }
 
