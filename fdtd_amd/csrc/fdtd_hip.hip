@@ -563,6 +563,7 @@ int adjoint_steps(FdtdAdjoint* a, int64_t nsteps, const T* curl, const T* plane,
   const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
   const dim3 grid(tiles * (unsigned)((p->d.nx + args.cx - 1) / args.cx)), block(bz * by);
   const dim3 ygrid((unsigned)((p->d.cs + 255) / 256));
+  const bool full = p->d.nz % bz == 0 && p->d.ny % by == 0;  // no thread past the (y, z) edges
   for (int64_t s = nsteps - 1; s >= 0; --s) {
     // each half step transposed in reverse: hard sources, then the periodic
     // copies (reverse registration order), then the main update
@@ -570,7 +571,10 @@ int adjoint_steps(FdtdAdjoint* a, int64_t nsteps, const T* curl, const T* plane,
     args.side = adj_sides<T>(a, true, a->parH);
     adj_mask_hard_sources<T>(p, args.lamH, true, st);
     adj_periodic<T>(p, args.lamH, true, st);
-    hipLaunchKernelGGL(k_adj_H<T>, grid, block, 0, st, args);
+    if (full)
+      hipLaunchKernelGGL((k_adj_H<T, true>), grid, block, 0, st, args);
+    else
+      hipLaunchKernelGGL((k_adj_H<T, false>), grid, block, 0, st, args);
     a->parH ^= 1;
     args.curl = curl + s * 3 * nd;
     args.side = adj_sides<T>(a, false, a->parE);

@@ -1273,7 +1273,11 @@ __global__ void __launch_bounds__(256) k_adj_zero(T* lam, const int64_t* cells, 
 // - ([j>=1] Y_z(j-1) - [j<=ny-2] Y_z(j)), and cyclically. The factor dcoef
 // comes from the transpose of the following E half step (k_adj_E leaves it);
 // after the update the objective's 2 E is added inside the objective box.
-template <typename T>
+// FULL: the tiles cover the (y, z) plane exactly (no thread past an edge), so
+// the stores need no guard; with a guard, the branch's untaken side leaves
+// the plane's loads unconsumed, and the wait inserted for the next plane's
+// register reuse then also waits for this plane's stores.
+template <typename T, bool FULL>
 __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
   const Dims d = a.d;
   const int bzs = a.bz_shift, bz = 1 << bzs, by = a.by;
@@ -1283,7 +1287,7 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
   // as in the main kernels: every load of a plane at once, threads past the
   // edge on the nearest cell (storing nothing), neighbour offset 0 at an edge
   const int k0 = tz * bz + (threadIdx.x & (bz - 1)), j0 = ty * by + (threadIdx.x >> bzs);
-  const bool act = k0 < d.nz && j0 < d.ny;
+  const bool act = FULL || (k0 < d.nz && j0 < d.ny);
   const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
   const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
   const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
@@ -1291,13 +1295,22 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
   const T zero = T(0), sc = a.sc;
   auto Y = [&](int c, int64_t x) { return (sc * a.inv_mu[c * cs + x]) * a.lamH[c * cs + x]; };
   int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
-  T yz_prev = zero, yy_prev = zero;  // Y at plane i-1
+  // inv_mu and lamH of components 1 and 2 at plane i-1, as loaded: Y is formed
+  // where it is used. A carried product is computed right after its loads,
+  // ahead of the branch that guards the stores, and the wave then waits for
+  // them before it issues the plane's other loads (two round trips a plane).
+  T mz_prev = zero, lz_prev = zero, my_prev = zero, ly_prev = zero;
   if (i0 > 0) {
-    yz_prev = Y(2, cell - sx);
-    yy_prev = Y(1, cell - sx);
+    mz_prev = a.inv_mu[2 * cs + cell - sx];
+    lz_prev = a.lamH[2 * cs + cell - sx];
+    my_prev = a.inv_mu[cs + cell - sx];
+    ly_prev = a.lamH[cs + cell - sx];
   }
   for (int i = i0; i < i1; ++i, cell += sx) {
-    const T y0 = Y(0, cell), y1 = Y(1, cell), y2 = Y(2, cell);
+    const T m1 = a.inv_mu[cs + cell], m2 = a.inv_mu[2 * cs + cell];
+    const T lh1 = a.lamH[cs + cell], lh2 = a.lamH[2 * cs + cell];
+    const T y0 = Y(0, cell), y1 = (sc * m1) * lh1, y2 = (sc * m2) * lh2;
+    const T yz_prev = (sc * mz_prev) * lz_prev, yy_prev = (sc * my_prev) * ly_prev;  // Y at plane i-1
     const T yy_km = Y(1, cell - zm), yx_km = Y(0, cell - zm);
     const T yz_jm = Y(2, cell - ym), yx_jm = Y(0, cell - ym);
     T e0 = a.lamE[cell], e1 = a.lamE[cs + cell], e2 = a.lamE[2 * cs + cell];
@@ -1334,8 +1347,10 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
       a.lamE[cs + cell] = e1;
       a.lamE[2 * cs + cell] = e2;
     }
-    yz_prev = y2;
-    yy_prev = y1;
+    mz_prev = m2;
+    lz_prev = lh2;
+    my_prev = m1;
+    ly_prev = lh1;
   }
 }
 
@@ -1414,7 +1429,9 @@ __global__ void __launch_bounds__(256) k_adj_periodic(T* lam, Dims d, int axis, 
   }
 }
 
-// YB: Y read from ybuf (k_adj_Y ran first) instead of gcoef lamE.
+// YB: Y read from ybuf (k_adj_Y ran first) instead of gcoef lamE. (A FULL
+// form as for k_adj_H is not used here: without the guard the compiler's
+// register reuse made the float kernel wait for the last plane's stores.)
 template <typename T, bool YB>
 __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   const Dims d = a.d;
@@ -1433,19 +1450,23 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   // Y of component c at cell x whose lamE is l
   auto Yc = [&](int c, int64_t x, T l) { return YB ? a.ybuf[c * cs + x] : a.gcoef[c * cs + x] * l; };
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
-  T yz_next = zero, yy_next = zero;  // Y at plane i+1
-  T lz_next = zero, ly_next = zero;  // lamE at plane i+1
+  // plane i+1, as loaded: lamE of components 1 and 2 and either their Y
+  // (ybuf) or their gcoef (Y = gcoef lamE is formed where it is used, for the
+  // reason given in k_adj_H)
+  T lz_next = zero, ly_next = zero, fz_next = zero, fy_next = zero;
   if (i1 < d.nx) {
     lz_next = L(2, cell + sx);
     ly_next = L(1, cell + sx);
-    yz_next = Yc(2, cell + sx, lz_next);
-    yy_next = Yc(1, cell + sx, ly_next);
+    fz_next = YB ? a.ybuf[2 * cs + cell + sx] : a.gcoef[2 * cs + cell + sx];
+    fy_next = YB ? a.ybuf[cs + cell + sx] : a.gcoef[cs + cell + sx];
   }
   for (int i = i1 - 1; i >= i0; --i, cell -= sx) {
     const T le0 = L(0, cell), le1 = L(1, cell), le2 = L(2, cell);
     const T lk0 = L(0, cell + zp), lk1 = L(1, cell + zp);
     const T lj0 = L(0, cell + yp), lj2 = L(2, cell + yp);
-    const T y0 = Yc(0, cell, le0), y1 = Yc(1, cell, le1), y2 = Yc(2, cell, le2);
+    const T f1 = YB ? a.ybuf[cs + cell] : a.gcoef[cs + cell], f2 = YB ? a.ybuf[2 * cs + cell] : a.gcoef[2 * cs + cell];
+    const T y0 = Yc(0, cell, le0), y1 = YB ? f1 : f1 * le1, y2 = YB ? f2 : f2 * le2;
+    const T yz_next = YB ? fz_next : fz_next * lz_next, yy_next = YB ? fy_next : fy_next * ly_next;  // Y at plane i+1
     const T yy_kp = Yc(1, cell + zp, lk1), yx_kp = Yc(0, cell + zp, lk0);
     const T yz_jp = Yc(2, cell + yp, lj2), yx_jp = Yc(0, cell + yp, lj0);
     const T l0 = a.lamH[cell], l1 = a.lamH[cs + cell], l2 = a.lamH[2 * cs + cell];
@@ -1476,8 +1497,8 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
           a.grad[c * nd + pd] += (double)sc * (double)le[c] * (double)a.curl[c * nd + pd];
       }
     }
-    yz_next = y2;
-    yy_next = y1;
+    fz_next = f2;
+    fy_next = f1;
     lz_next = le2;
     ly_next = le1;
   }
