@@ -563,7 +563,10 @@ int adjoint_steps(FdtdAdjoint* a, int64_t nsteps, const T* curl, const T* plane,
   const unsigned tiles = (unsigned)(((p->d.nz + bz - 1) / bz) * ((p->d.ny + by - 1) / by));
   const dim3 grid(tiles * (unsigned)((p->d.nx + args.cx - 1) / args.cx)), block(bz * by);
   const dim3 ygrid((unsigned)((p->d.cs + 255) / 256));
-  const bool full = p->d.nz % bz == 0 && p->d.ny % by == 0;  // no thread past the (y, z) edges
+  // no thread past the (y, z) edges: k_adj_H without its store guard
+  // (FDTD_HIP_ADJ_FULL=0 keeps the guard, to check that both forms agree)
+  const char* full_env = getenv("FDTD_HIP_ADJ_FULL");
+  const bool full = p->d.nz % bz == 0 && p->d.ny % by == 0 && !(full_env && atoi(full_env) == 0);
   for (int64_t s = nsteps - 1; s >= 0; --s) {
     // each half step transposed in reverse: hard sources, then the periodic
     // copies (reverse registration order), then the main update

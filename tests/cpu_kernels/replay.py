@@ -21,6 +21,8 @@ the device library gets. Prints one JSON line per case: {"case", "ok",
                       NaN / an infinity written into E / H
   adjoint:<name>      the gradient of a gradient scenario (record mode)
                       against the reference-autograd fixture, 1e-10
+  adjoint_full:<backend>  the transposed H kernel without its store guard
+                      against the guarded form, bit for bit
   batch:<seed>        grids of uneven shapes stepped by fdtd_batch_run
                       (direct launches, graph replays and a persistent
                       launch) and by fdtd_batch_run_static, bit for bit
@@ -195,6 +197,39 @@ def case_adjoint(name):
     return errs
 
 
+def case_adjoint_full(backend):
+    """The transposed H kernel without its store guard (tiles covering the
+    (y, z) plane) against the guarded form (FDTD_HIP_ADJ_FULL=0): the same
+    gradient bit for bit."""
+    import adjoint_scenarios as A
+    from fdtd_amd.adjoint import IntensityGradient
+
+    sc = A.FULL_TILES
+    out = []
+    for full in (None, "0"):
+        if full is None:
+            os.environ.pop("FDTD_HIP_ADJ_FULL", None)
+        else:
+            os.environ["FDTD_HIP_ADJ_FULL"] = full
+        try:
+            fdtd.set_backend(backend)
+            perm = A.design_permittivity(sc)
+            grid, design = A.build(fdtd, sc, torch.from_numpy(perm))
+            prob = IntensityGradient(grid, design, sc["plane"], sc["steps"], checkpoint="record")
+            out.append(prob.gradient(torch.from_numpy(perm)))
+        finally:
+            os.environ.pop("FDTD_HIP_ADJ_FULL", None)
+    (J0, g0), (J1, g1) = out
+    errs = []
+    if J0 != J1:
+        errs.append(f"J {J0!r} against {J1!r}")
+    if not torch.equal(g0, g1):
+        errs.append("gradients differ")
+    if not bool(torch.isfinite(g0).all()) or float(g0.abs().max()) == 0.0:
+        errs.append("gradient not finite or all zero")
+    return errs
+
+
 def case_nonfinite(name):
     """Grid.has_nonfinite on a golden scenario: False after its run, True
     after a NaN or an infinity is written into one E or H value."""
@@ -289,7 +324,7 @@ def case_batch(seed):
 
 CASES = {"fixture": case_fixture, "random": case_random, "random_complex": case_random_complex,
          "compensated": case_compensated, "batch": case_batch, "adjoint": case_adjoint,
-         "nonfinite": case_nonfinite}
+         "adjoint_full": case_adjoint_full, "nonfinite": case_nonfinite}
 
 
 def main(argv):

@@ -89,6 +89,11 @@ PA = dict(
 # the same features on TINY (central differences over every design voxel)
 TINY_PA = dict(TINY, name="tiny_pa", periodic_z=True, aniso=((slice(13, 15), slice(4, 14), slice(0, 14)), 17))
 
+# tiles that cover the (y, z) plane exactly (nz = 32: 32-cell z rows, ny = 48:
+# 8 rows of 256-thread tiles), for the transposed H kernel's guard-free form
+FULL_TILES = dict(G5, name="full_tiles", shape=(40, 48, 32), steps=120, source=(10, slice(14, 34), 16),
+                  plane=(30, slice(20, 28), slice(8, 16)))
+
 
 def aniso_inverse(sc):
     """The overwritten inverse permittivity of the anisotropic object:

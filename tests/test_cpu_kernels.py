@@ -9,7 +9,8 @@ oracle; the compensated form stays within its bound; batched runs (direct,
 graph replay, persistent, and the static-table entry point) equal the grids
 stepped alone; the transposed steps give the reference-autograd gradients of
 both gradient fixtures (all PML faces; periodic z with an anisotropic
-object); the non-finite check flags a NaN or an infinity in E or H and
+object), and the guard-free transposed H kernel of tiles that cover the
+(y, z) plane equals the guarded one bit for bit; the non-finite check flags a NaN or an infinity in E or H and
 nothing else.
 Both tile shapes of the main kernels run: the default (flat tiles for these
 grids, whose nz is not a multiple of 64) and row tiles forced with
@@ -35,7 +36,7 @@ def _cases():
     out += [f"random:{s}" for s in range(6)]
     out += [f"random_complex:{s}" for s in range(6)]
     out += ["compensated:g2", "compensated:periodic_x_first", "batch:1", "batch:2", "adjoint:g5", "adjoint:pa",
-            "nonfinite:g2", "nonfinite:complex_object"]
+            "adjoint_full:hip.float64", "adjoint_full:hip.float32", "nonfinite:g2", "nonfinite:complex_object"]
     return out
 
 

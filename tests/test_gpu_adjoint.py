@@ -11,6 +11,8 @@
     128x128x64 / 64x64x16-design / 2000-step case, in float64
   * the float32 gradient of the 128x128x64 case against the float64 one,
     error stated
+  * tiles covering the (y, z) plane: the guard-free transposed H kernel
+    against the guarded one, bit for bit
 
 Tolerances: against the reference autograd 1e-10 relative L2 (the transposes
 are exact; only rounding differs); against central differences 1e-4
@@ -204,6 +206,22 @@ def test_c5_float32_gradient_error_against_float64():
     eg = rel(g32.cpu().numpy(), g64.cpu().numpy())
     _say(f"[gpu] adjoint c5 float32 vs float64: J rel {eJ:.2e}, grad rel L2 {eg:.2e}")
     assert eJ <= 1e-4 and eg <= 1e-3
+
+
+@pytest.mark.parametrize("backend", ["hip.float64", "hip.float32"])
+def test_guard_free_transposed_kernel_matches_guarded(backend, monkeypatch):
+    """Tiles that cover the (y, z) plane exactly run the transposed H kernel
+    without its store guard; FDTD_HIP_ADJ_FULL=0 keeps the guard. Same
+    gradient bit for bit."""
+    sc = A.FULL_TILES
+    prob, perm = _problem(sc, backend, checkpoint="record")
+    J0, g0 = prob.gradient(torch.from_numpy(perm))
+    monkeypatch.setenv("FDTD_HIP_ADJ_FULL", "0")
+    J1, g1 = prob.gradient(torch.from_numpy(perm))
+    _say(f"[gpu] adjoint full tiles {backend}: J {J0:.6e}")
+    assert J0 == J1
+    assert torch.equal(g0, g1)
+    assert bool(torch.isfinite(g0).all()) and float(g0.abs().max()) > 0
 
 
 def test_transposed_launch_shape_leaves_gradient_unchanged():
