@@ -57,26 +57,29 @@ PML_KEYS = lambda t: [(slice(0, t), slice(None), slice(None)), (slice(-t, None),
 PALETTE = "auto"  # --palette: how the grid material is read (auto | on | off)
 
 
-def build_grid(fdtd, n=256, pml=16):
+def build_grid(fdtd, n=256, pml=16, palette=None):
     eps = sphere_permittivity(n)
     grid = fdtd.Grid((n, n, n), grid_spacing=50e-9, permittivity=eps, courant_number=3 ** -0.5)
-    grid.material_palette = {"auto": "auto", "on": True, "off": False}[PALETTE]
+    grid.material_palette = {"auto": "auto", "on": True, "off": False}[palette or PALETTE]
     for k in PML_KEYS(pml):
         grid[k] = fdtd.PML()
     grid[n // 2, n // 2, n // 2] = fdtd.PointSource(period=1550e-9 / 299792458.0, pulse=True)
     return grid
 
 
-def algorithmic_bytes_per_half_step(n, pml, itemsize, compensated=False):
+def algorithmic_bytes_per_half_step(n, pml, itemsize, compensated=False, palette=False):
     """12 values per cell (read field, other field, material; write field) plus
     4 per (face, slab cell) (read and write the two psi components). The
     compensated form stores a field value in 5 bytes (float + int8), reads the
-    material as a 1-byte palette index and keeps psi in float64."""
+    material as a 1-byte palette index and keeps psi in float64; with a
+    palette alone the material is that 1-byte index."""
     cells = n ** 3
     if compensated:
         slab = pml * n * n
         return cells * (9 * 5 + 1) + 6 * slab * 4 * 8
     slab = pml * n * n
+    if palette:
+        return cells * (9 * itemsize + 1) + 6 * slab * 4 * itemsize
     return cells * 12 * itemsize + 6 * slab * 4 * itemsize
 
 
@@ -156,13 +159,15 @@ def field_errors(got, ref):
             "pointwise_rel_floor_1e-3max": float((d / ref.abs().clamp_min(1e-3 * mx)).max())}
 
 
-def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps, keep_fields=False):
+def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps, keep_fields=False, palette=None):
     """Mcells/s over `steps` timed steps (median of WINDOWS windows) and the
     roofline of the dominant kernel; with keep_fields, also the final E and H
-    (float64, decoded) for error figures."""
+    (float64, decoded) for error figures. `palette` ("on"/"off") overrides
+    --palette for this run."""
     fdtd.set_backend(f"hip.{dtype}")
-    grid = build_grid(fdtd, n, pml)
+    grid = build_grid(fdtd, n, pml, palette)
     grid.run(warmup, progress_bar=False)
+    pal = grid._palette_wanted() and grid._material_palette() is not None  # palette in use (<= 256 materials)
     torch.cuda.synchronize()
     stream = torch.cuda.current_stream()
     windows = []
@@ -178,7 +183,8 @@ def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps, keep_fields=
     ms = float(np.median(windows))
     ms_step = ms / steps
     itemsize = 8 if dtype == "float64" else 4
-    half_bytes = algorithmic_bytes_per_half_step(n, pml, itemsize, compensated=dtype == "float32c")
+    half_bytes = algorithmic_bytes_per_half_step(n, pml, itemsize, compensated=dtype == "float32c",
+                                                 palette=pal and dtype != "float32c")
     fields = None
     if keep_fields:
         fields = torch.cat([grid.full_precision("E").reshape(-1), grid.full_precision("H").reshape(-1)])
@@ -190,6 +196,8 @@ def measure(torch, fdtd, dtype, n, pml, steps, warmup, kernel_reps, keep_fields=
     dom = max(per_launch, key=per_launch.get)
     achieved = half_bytes / (per_launch[dom] / 1e3) / 1e9
     ctype = {"float64": "double", "float32": "float"}.get(dtype, "compensated")
+    if pal and dtype != "float32c":
+        ctype += ", palette"
     total_steps = warmup + WINDOWS * steps
     del grid, plan
     torch.cuda.empty_cache()
@@ -503,7 +511,8 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=6)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--kernel-reps", type=int, default=50)
-    ap.add_argument("--no-fp32", action="store_true", help="skip the float32 measurement")
+    ap.add_argument("--no-fp32", action="store_true",
+                    help="skip the float32, compensated and float64-palette legs of the plain line")
     ap.add_argument("--form", default="plain", choices=["plain", "precision", "adjoint", "batch"],
                     help="plain: the 256^3 headline line; precision: float32 forms vs float64; "
                          "adjoint: gradient/forward time on the differentiation configuration; "
@@ -556,6 +565,14 @@ def main():
             m = measure(torch, fdtd, name, n, pml, args.steps, args.warmup, args.kernel_reps, keep_fields=True)
             m["error_vs_f64"] = field_errors(m.pop("fields"), main["fields"])
             others[name] = m
+        if PALETTE == "auto":
+            # a specialised float64 kernel: the grid materials as a 1-byte
+            # palette index into their float64 values (<= 256 distinct), the
+            # same arithmetic on the same values, so the fields must be equal
+            m = measure(torch, fdtd, "float64", n, pml, args.steps, args.warmup, args.kernel_reps, keep_fields=True,
+                        palette="on")
+            m["bit_identical"] = bool(torch.equal(m.pop("fields"), main["fields"]))
+            others["float64_palette"] = m
     main.pop("fields", None)
     copy_gbs = copy_bandwidth(torch)
 
@@ -599,6 +616,17 @@ def main():
                     "speedup_vs_f64": round(o["mcells"] / main["mcells"], 3),
                     "error_vs_f64": dict(o["error_vs_f64"], steps=o["steps_run"],
                                          against="hip.float64 run of the same workload and steps")}
+    if "float64_palette" in others:
+        o = others["float64_palette"]
+        out["fp64_palette"] = {
+            "value": round(o["mcells"], 1), "unit": "Mcells/s", "ms_per_step": round(o["ms_step"], 4),
+            "windows_ms_per_step": o["windows_ms_per_step"],
+            "storage": "f64 fields, CPML state and arithmetic; grid materials as a 1-byte index into a float64 "
+                       "palette (a specialised kernel: at most 256 distinct materials), roofline against its own "
+                       "reduced bytes",
+            "step_hbm_roofline_frac": round(o["step_frac"], 4), "roofline": o["roofline"],
+            "speedup_vs_f64": round(o["mcells"] / main["mcells"], 3),
+            "bit_identical_to_f64": o["bit_identical"]}
     if not args.no_adjoint:
         out["adjoint"] = adjoint_measure("float32")
     if not args.no_batch:
