@@ -17,14 +17,17 @@ rest (E = H = CPML state = 0, time index 0), as a fresh reference grid does.
 Memory. The gradient needs, for every step, curl H inside the design box and
 E inside the objective box. ``checkpoint="record"`` keeps them for all T
 steps (one forward pass plus one reverse pass). ``checkpoint="sqrt"`` keeps
-a copy of the whole state (E, H, every CPML psi) every K = ceil(sqrt(T))
-steps instead; the reverse pass restores each checkpoint, steps its segment
-forward again recording K steps, and runs the transposed steps over it (two
-forward passes plus one reverse pass, O(sqrt(T)) states of memory). There are
-2 to RECORD_BUFFERS record buffers of K steps (a quarter of the checkpoints'
-memory at most); the last segments, one per buffer, are recorded by the first
-forward pass into the buffers the reverse pass reads, so they are neither
-checkpointed nor recomputed.
+a copy of the whole state (E, H, every CPML psi) every K steps instead; the
+reverse pass restores each checkpoint, steps its segment forward again
+recording K steps, and runs the transposed steps over it. There are B record
+buffers of K steps; the last B segments, one per buffer, are recorded by the
+first forward pass into the buffers the reverse pass reads, so they are
+neither checkpointed nor recomputed. K and B come from the memory: the
+budget is that of checkpoints every ceil(sqrt(T)) steps plus a quarter,
+O(sqrt(T)) states, and within it the pair that records the most steps
+(records are much smaller than a state when the design and objective boxes
+are small, so fewer, longer segments and more buffers pay; see
+``_sqrt_schedule``).
 ``"auto"`` picks "record" when the records fit in a quarter of the free
 device memory.
 """
@@ -38,14 +41,14 @@ from . import _native
 from .boundaries import PML, PeriodicBoundary
 from .objects import AbsorbingObject, AnisotropicObject, Object
 
-# checkpointed mode: at most RECORD_BUFFERS record buffers of K steps (the
-# last ones' segments are recorded by the forward pass, the others recomputed
-# into them in turn; together they take at most a quarter of the checkpoints'
-# memory, and there are at least two), and x planes per workgroup of the
-# transposed kernels running beside the recomputation. Measured on MI355X at
-# 128x128x64, 2000 steps: 10 buffers (the memory bound there) 2.89x the
-# forward, 8 2.94x, 6 2.96x, 4 2.98x; 2 planes beat 1 and 4.
+# checkpointed mode: at most RECORD_BUFFERS record buffers (the last ones'
+# segments are recorded by the forward pass, the others recomputed into them
+# in turn), SEGMENT_STEPS a fixed segment length instead of the one chosen
+# from the memory (None), and x planes per workgroup of the transposed
+# kernels running beside the recomputation (2 beat 1 and 4 on MI355X at
+# 128x128x64, 2000 steps).
 RECORD_BUFFERS = 16
+SEGMENT_STEPS = None
 SQRT_MODE_CX = 2
 
 
@@ -226,14 +229,44 @@ class IntensityGradient:
             t.zero_()
         self.grid.time_steps_passed = 0
 
-    def _record_buffers(self, K):
-        """Record buffers of the checkpointed mode: up to RECORD_BUFFERS, at
-        least 2, together at most a quarter of the checkpoints' memory."""
+    def _sqrt_schedule(self):
+        """(K, B) of the checkpointed mode: segments of K steps and B record
+        buffers. With C the bytes of a state (E, H, every CPML psi) and r of
+        one step's records (curl in the design box, E in the objective box),
+        its memory is (ceil(T/K) - B) checkpoints and B buffers of K steps,
+        and the recomputation covers the T - B K steps that the forward pass
+        does not record. Segments are K0 = ceil(sqrt(T)) to 2 K0 steps long
+        and the memory at most that of the classic scheme's checkpoints (every
+        K0 steps) plus a quarter, 1.25 ceil(T/K0) C: O(sqrt(T)) either way.
+        Within that, the pair with the most recorded steps (2 <= B <=
+        RECORD_BUFFERS; ties: less memory); K0 and 2 buffers when none fits.
+        SEGMENT_STEPS fixes K."""
+        T = self.steps
         esz = self.grid._E.element_size()
-        per_buffer = K * 3 * (self.nd + self.np) * esz
-        checkpoint = sum(t.numel() * t.element_size() for t in self._state())
-        nseg = -(-self.steps // K)
-        return max(2, min(RECORD_BUFFERS, (nseg * checkpoint) // (4 * per_buffer)))
+        r = 3 * (self.nd + self.np) * esz
+        C = sum(t.numel() * t.element_size() for t in self._state())
+        K0 = max(1, math.isqrt(T - 1) + 1)
+        budget = 1.25 * (-(-T // K0)) * C
+        if SEGMENT_STEPS is None:
+            Ks = np.arange(K0, min(2 * K0, T) + 1, dtype=np.float64)
+        else:
+            Ks = np.array([float(SEGMENT_STEPS)])
+        nseg = np.ceil(T / Ks)
+        best = None  # (recorded steps, -memory, K, B)
+        for B in range(2, max(2, RECORD_BUFFERS) + 1):
+            mem = np.maximum(nseg - B, 0) * C + np.minimum(B, nseg) * Ks * r
+            rec = np.minimum(B * Ks, T)
+            ok = mem <= budget
+            if not ok.any():
+                continue
+            i = np.lexsort((mem[ok], -rec[ok]))[0]  # most recorded steps, then least memory
+            cand = (float(rec[ok][i]), -float(mem[ok][i]), int(Ks[ok][i]), B)
+            if best is None or cand[:2] > best[:2]:
+                best = cand
+        if best is None:
+            return (int(SEGMENT_STEPS) if SEGMENT_STEPS else K0), 2
+        K, B = best[2], best[3]
+        return K, int(min(B, -(-T // K)))
 
     def _mode(self):
         if self.checkpoint != "auto":
@@ -313,7 +346,13 @@ class IntensityGradient:
         stream = torch.cuda.current_stream().cuda_stream
         T = self.steps
         mode = self._mode() if gradient else "sqrt"
-        K = T if mode == "record" else max(1, math.isqrt(T - 1) + 1)
+        sqrt_grad = gradient and mode == "sqrt"
+        if mode == "record":
+            K, nbuf = T, 1
+        elif sqrt_grad:
+            K, nbuf = self._sqrt_schedule()
+        else:
+            K, nbuf = max(1, math.isqrt(T - 1) + 1), 1
         gcoef, dcoef = self._coefficients()
         lamE = torch.zeros_like(g._E)
         lamH = torch.zeros_like(g._H)
@@ -324,8 +363,6 @@ class IntensityGradient:
         # recomputation of earlier segments fills some while the transposed
         # steps of a later one read another; fewer and longer workgroups of
         # the transposed kernels leave room for the recomputation beside them
-        sqrt_grad = gradient and mode == "sqrt"
-        nbuf = self._record_buffers(K) if sqrt_grad else 1
         adj.set_launch(SQRT_MODE_CX if sqrt_grad else 0)
         curl = [torch.empty((K, 3, self.nd), dtype=dt, device=dev) for _ in range(nbuf)]
         plane = [torch.empty((K, 3, self.np), dtype=dt, device=dev) for _ in range(nbuf)]

@@ -304,28 +304,49 @@ def test_checkpointed_reverse_schedule():
         _check_reverse_schedule(too_far_ahead(20, 4), 20, 4)
 
 
-def test_checkpointed_record_buffer_count():
-    """IntensityGradient._record_buffers: between 2 and RECORD_BUFFERS buffers
-    of K steps, together at most a quarter of the checkpoints' memory."""
+def test_checkpointed_schedule(monkeypatch):
+    """IntensityGradient._sqrt_schedule: segments of ceil(sqrt(T)) to twice
+    that many steps and 2 to RECORD_BUFFERS record buffers, within the memory
+    of the classic schedule's checkpoints plus a quarter; among those the pair
+    that records the most steps (checked by brute force), the classic
+    schedule with two buffers when none fits, SEGMENT_STEPS fixing the length."""
     from types import SimpleNamespace
 
-    from fdtd_amd.adjoint import RECORD_BUFFERS, IntensityGradient
+    import fdtd_amd.adjoint as A
 
     E = torch.zeros((3, 128, 128, 64), dtype=torch.float32)
     psi = [torch.zeros(2 * 12 * 128 * 64, dtype=torch.float32) for _ in range(12)]
-
-    def count(nd, steps=2000, K=45):
-        fake = SimpleNamespace(grid=SimpleNamespace(_E=E), steps=steps, nd=nd, np=256,
-                               _state=lambda: [E, E] + psi)
-        return IntensityGradient._record_buffers(fake, K)
-
-    n_small = count(64 * 64 * 16)                        # the differentiation workload: small records
-    assert 8 <= n_small <= RECORD_BUFFERS
-    assert count(128 * 128 * 64) == 2                    # a whole-grid design: records as large as a state
-    n = count(128 * 128 * 16)
-    assert 2 <= n <= RECORD_BUFFERS
     state = sum(t.numel() * 4 for t in [E, E] + psi)
-    assert n * 45 * 3 * (128 * 128 * 16 + 256) * 4 <= 45 * state / 4
+    T, K0 = 2000, 45
+    budget = 1.25 * K0 * state
+
+    def sched(nd):
+        fake = SimpleNamespace(grid=SimpleNamespace(_E=E), steps=T, nd=nd, np=256, _state=lambda: [E, E] + psi)
+        return A.IntensityGradient._sqrt_schedule(fake)
+
+    def memory(K, B, nd):
+        nseg = -(-T // K)
+        return max(nseg - B, 0) * state + min(B, nseg) * K * 3 * (nd + 256) * 4
+
+    # the differentiation workload (64x64x16 design): small records
+    nd = 64 * 64 * 16
+    K, B = sched(nd)
+    assert K0 <= K <= 2 * K0 and 2 <= B <= A.RECORD_BUFFERS
+    assert memory(K, B, nd) <= budget
+    assert -(-T // K) > B  # still checkpointed and recomputed
+    best = max(min(B2 * K2, T) for B2 in range(2, A.RECORD_BUFFERS + 1) for K2 in range(K0, 2 * K0 + 1)
+               if memory(K2, B2, nd) <= budget)
+    assert min(B * K, T) == best
+    assert T - B * K <= T // 3
+    # a larger design: fewer buffers fit
+    K2, B2 = sched(128 * 128 * 16)
+    assert memory(K2, B2, 128 * 128 * 16) <= budget and 2 <= B2 < B
+    # a whole-grid design: records as large as half a state, nothing fits
+    assert sched(128 * 128 * 64) == (K0, 2)
+    # a fixed segment length
+    monkeypatch.setattr(A, "SEGMENT_STEPS", K0)
+    K3, B3 = sched(nd)
+    assert K3 == K0 and 2 <= B3 <= A.RECORD_BUFFERS and memory(K3, B3, nd) <= budget
 
 
 def test_compensated_records_and_host_writes():
