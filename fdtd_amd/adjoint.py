@@ -45,11 +45,12 @@ from .objects import AbsorbingObject, AnisotropicObject, Object
 # segments are recorded by the forward pass, the others recomputed into them
 # in turn), SEGMENT_STEPS a fixed segment length instead of the one chosen
 # from the memory (None), and x planes per workgroup of the transposed
-# kernels running beside the recomputation (2 beat 1 and 4 on MI355X at
-# 128x128x64, 2000 steps).
+# kernels running beside the recomputation (0: the plan's own; on MI355X at
+# 128x128x64, 2000 steps, where that is 1: 1 and 3 2.45x the forward, 2
+# 2.52x, 4 2.56x).
 RECORD_BUFFERS = 16
 SEGMENT_STEPS = None
-SQRT_MODE_CX = 2
+SQRT_MODE_CX = 0
 
 
 def _box_of(grid, key):
