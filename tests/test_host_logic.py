@@ -304,6 +304,21 @@ def test_checkpointed_reverse_schedule():
         _check_reverse_schedule(too_far_ahead(20, 4), 20, 4)
 
 
+def test_bench_algorithmic_bytes():
+    """bench.algorithmic_bytes_per_half_step at 256^3 with 16-cell PML: the
+    per-step figures of the measurement plan (24 values per cell plus 8 per
+    face slab cell: 3,623,878,656 B in float64, 1,811,939,328 B in float32);
+    the palette form reads a 1-byte material index instead of 3 values; the
+    compensated form 9 values of 5 bytes and the index, float64 psi."""
+    import bench
+
+    hb = bench.algorithmic_bytes_per_half_step
+    assert 2 * hb(256, 16, 8) == 3_623_878_656
+    assert 2 * hb(256, 16, 4) == 1_811_939_328
+    assert hb(256, 16, 8) - hb(256, 16, 8, palette=True) == 256 ** 3 * (3 * 8 - 1)
+    assert hb(256, 16, 4, compensated=True) == 256 ** 3 * 46 + 6 * 16 * 256 * 256 * 4 * 8
+
+
 def test_checkpointed_schedule(monkeypatch):
     """IntensityGradient._sqrt_schedule: segments of ceil(sqrt(T)) to twice
     that many steps and 2 to RECORD_BUFFERS record buffers, within the memory
