@@ -1157,10 +1157,12 @@ __device__ __forceinline__ int64_t slab_step(const Dims& d, int t) {
 // The faces normal to AX at one cell of k_adj_H (see there): the face's own
 // psi transpose and the difference terms of this cell and of the face cell
 // one step back along AX (which holds the Y values yb0, yb1). The face's
-// axis is a compile-time constant, so components and signs are too.
+// axis is a compile-time constant, so components and signs are too. t_lo and
+// t_hi are the slab thicknesses of the two sides (0: no face), read once per
+// kernel, so a cell outside every slab reads nothing of the face table.
 template <int AX, typename T>
-__device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int i, int j, int k, T yc0, T yc1, T yb0,
-                                           T yb1, T e[3]) {
+__device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int t_lo, int t_hi, int i, int j, int k, T yc0,
+                                           T yc1, T yb0, T yb1, T e[3]) {
   constexpr int C0 = AX == 0 ? 1 : 0, C1 = AX == 2 ? 1 : 2;
   constexpr int S0 = 3 - AX - C0, S1 = 3 - AX - C1;
   // phi_H = psi_{C+1} - psi_{C+2} enters H' with a minus sign
@@ -1170,10 +1172,10 @@ __device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int i, int j, i
   const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
 #pragma nounroll
   for (int side = 0; side < 2; ++side) {
-    const AdjSide<T>& fd = a.side[AX * 2 + side];
-    const int t = fd.t;
+    const int t = side ? t_hi : t_lo;
     const int l = side ? g - (n - t) : g;
     if (t == 0 || l < 0 || l > t) continue;  // neither this cell nor the one back is in the slab
+    const AdjSide<T>& fd = a.side[AX * 2 + side];
     const gptr<const T> pin = gp(fd.pin);
     const gptr<T> pout = gp(fd.pout);
     const int64_t off = slab_off<AX>(d, t, i, j, k, l), slab = fd.slab;
@@ -1205,8 +1207,8 @@ __device__ __forceinline__ void adj_faces_H(const AdjArgs<T>& a, int i, int j, i
 // one step ahead along AX. lc0, lc1 are lamE of the face's components C0, C1
 // at this cell, la0, la1 at the cell ahead (already loaded by the caller).
 template <int AX, typename T>
-__device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int i, int j, int k, int64_t cell, T lc0, T lc1,
-                                           T la0, T la1, T h[3]) {
+__device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int t_lo, int t_hi, int i, int j, int k, int64_t cell,
+                                           T lc0, T lc1, T la0, T la1, T h[3]) {
   constexpr int C0 = AX == 0 ? 1 : 0, C1 = AX == 2 ? 1 : 2;
   constexpr int S0 = 3 - AX - C0, S1 = 3 - AX - C1;
   // phi_E = psi_{C+1} - psi_{C+2} enters E' with a plus sign
@@ -1220,10 +1222,10 @@ __device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int i, int j, i
   auto K = [&](int c, int64_t x) { return sc * a.inv_eps[c * cs + x]; };
 #pragma nounroll
   for (int side = 0; side < 2; ++side) {
-    const AdjSide<T>& fd = a.side[AX * 2 + side];
-    const int t = fd.t;
+    const int t = side ? t_hi : t_lo;
     const int l = side ? g - (n - t) : g;
     if (t == 0 || l < -1 || l >= t) continue;  // neither this cell nor the one ahead is in the slab
+    const AdjSide<T>& fd = a.side[AX * 2 + side];
     const gptr<const T> pin = gp(fd.pin);
     const gptr<T> pout = gp(fd.pout);
     const int64_t off = slab_off<AX>(d, t, i, j, k, l), slab = fd.slab;
@@ -1294,6 +1296,9 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
   const int64_t zm = k > 0 ? 1 : 0, ym = j > 0 ? sy : 0;
   const T zero = T(0), sc = a.sc;
   auto Y = [&](int c, int64_t x) { return (sc * a.inv_mu[c * cs + x]) * a.lamH[c * cs + x]; };
+  int ts[6];  // slab thickness of each side's face (0: none), read once
+#pragma unroll
+  for (int q = 0; q < 6; ++q) ts[q] = a.side[q].t;
   int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
   // inv_mu and lamH of components 1 and 2 at plane i-1, as loaded: Y is formed
   // where it is used. A carried product is computed right after its loads,
@@ -1329,9 +1334,9 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
     if (act) {
       {
         T e[3] = {e0, e1, e2};
-        adj_faces_H<0>(a, i, j, k, y1, y2, yy_prev, yz_prev, e);
-        adj_faces_H<1>(a, i, j, k, y0, y2, yx_jm, yz_jm, e);
-        adj_faces_H<2>(a, i, j, k, y0, y1, yx_km, yy_km, e);
+        adj_faces_H<0>(a, ts[0], ts[1], i, j, k, y1, y2, yy_prev, yz_prev, e);
+        adj_faces_H<1>(a, ts[2], ts[3], i, j, k, y0, y2, yx_jm, yz_jm, e);
+        adj_faces_H<2>(a, ts[4], ts[5], i, j, k, y0, y1, yx_km, yy_km, e);
         e0 = e[0];
         e1 = e[1];
         e2 = e[2];
@@ -1449,6 +1454,9 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   auto L = [&](int c, int64_t x) { return a.lamE[c * cs + x]; };
   // Y of component c at cell x whose lamE is l
   auto Yc = [&](int c, int64_t x, T l) { return YB ? a.ybuf[c * cs + x] : a.gcoef[c * cs + x] * l; };
+  int ts[6];  // slab thickness of each side's face (0: none), read once
+#pragma unroll
+  for (int q = 0; q < 6; ++q) ts[q] = a.side[q].t;
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
   // plane i+1, as loaded: lamE of components 1 and 2 and either their Y
   // (ybuf) or their gcoef (Y = gcoef lamE is formed where it is used, for the
@@ -1478,9 +1486,9 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
     if (act) {
       {
         T h[3] = {h0, h1, h2};
-        adj_faces_E<0>(a, i, j, k, cell, le1, le2, ly_next, lz_next, h);
-        adj_faces_E<1>(a, i, j, k, cell, le0, le2, lj0, lj2, h);
-        adj_faces_E<2>(a, i, j, k, cell, le0, le1, lk0, lk1, h);
+        adj_faces_E<0>(a, ts[0], ts[1], i, j, k, cell, le1, le2, ly_next, lz_next, h);
+        adj_faces_E<1>(a, ts[2], ts[3], i, j, k, cell, le0, le2, lj0, lj2, h);
+        adj_faces_E<2>(a, ts[4], ts[5], i, j, k, cell, le0, le1, lk0, lk1, h);
         h0 = h[0];
         h1 = h[1];
         h2 = h[2];
