@@ -1079,20 +1079,22 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     }
     const int bzs = lanes_shift(p->d.nz);
     const int by = std::max(1, p->block_threads >> bzs);
-    // flat tiles where z-rows of whole wavefronts would leave lanes idle
-    // (nz not a multiple of 64); planes of 2^31 cells and more keep row tiles
+    // flat tiles (a workgroup's cells consecutive in the (y, z) plane): no
+    // lanes idle on uneven z edges, and a workgroup streams one contiguous
+    // run of each array; measured on MI355X at 256^3 they beat tiles of four
+    // 64-cell z-rows by 4-9 % in every storage. Planes of 2^31 cells and more
+    // keep row tiles.
     const int64_t plane = (int64_t)p->d.ny * p->d.nz;
-    p->flat = p->d.nz % 64 != 0;
+    p->flat = true;
     if (const char* env = getenv("FDTD_HIP_FLAT")) p->flat = atoi(env) != 0;
     p->flat = p->flat && plane < ((int64_t)1 << 31);
     const int64_t tiles = plane_tiles(p->d, bzs, by, p->flat ? p->block_threads : 0);
-    // measured on MI355X: plain storage is fastest at 2 planes per workgroup
-    // from 256^3 up to 512x512x384 (14 % faster there than at 12) and at 1 on
-    // grids of ~1M cells; the compensated kernels at 64 x-chunks (4 planes at
-    // 256^3, 5 % faster than 2; 8 at 512x512x384)
+    // x planes per workgroup, measured on MI355X with flat tiles at 256^3 and
+    // 512x512x384: float64 is fastest at 1 (3-4 % ahead of 2), float32 and the
+    // compensated storage at 2 (5-6 % ahead of 1 and 3); every storage at 1 on
+    // grids of ~1M cells
     const int64_t work = tiles * p->d.nx;
-    int64_t cx = std::min<int64_t>(2, work / 32768);
-    if (p->dtype == FDTD_F32C && work >= 65536) cx = std::max<int64_t>(2, std::min<int64_t>(8, p->d.nx / 64));
+    int64_t cx = p->dtype == FDTD_F64 ? 1 : std::min<int64_t>(2, work / 32768);
     if (const char* env = getenv("FDTD_HIP_CX")) cx = atoi(env);
     if (const char* env = getenv("FDTD_HIP_NT")) p->nontemporal = atoi(env) != 0;
     // cyclic dealing pays once the fields outgrow the caches (16.7M cells = 256^3)
