@@ -6,6 +6,8 @@
   * the 256^3 configuration with 16-cell PML: a few steps against the oracle,
     plus size-independent properties at full size (exact linearity)
   * float32: error against the float64 fixture, stated and bounded
+  * the launch-shape overrides (tiles, x planes per workgroup, chunk
+    dealing, workgroup size, non-temporal accesses): bit-identical results
 
 Tolerances: the float64 bar is <= 1e-12 relative L2 on all fields; the
 kernels are written to reproduce numpy's rounding sequence, so the tests
@@ -141,6 +143,51 @@ def test_save_data_writes_reference_readings(golden, tmp_path, monkeypatch):
                 assert got.ndim == (3 if isinstance(c, fdtd.LineDetector) else 5)
                 assert got.shape[0] == grid.time_steps_passed
                 _assert_same(got, g[f"out/{i}/{F}"], f"saved {c.name} ({F})")
+
+
+LAUNCH_OVERRIDES = ["FDTD_HIP_FLAT", "FDTD_HIP_CX", "FDTD_HIP_CYCLIC", "FDTD_HIP_BLOCK", "FDTD_HIP_NT"]
+LAUNCH_SHAPES = [
+    dict(FDTD_HIP_FLAT="0"),
+    dict(FDTD_HIP_CX="3"),
+    dict(FDTD_HIP_CX="2", FDTD_HIP_CYCLIC="1"),
+    dict(FDTD_HIP_CYCLIC="0", FDTD_HIP_BLOCK="512"),
+    dict(FDTD_HIP_BLOCK="1024", FDTD_HIP_FLAT="0"),
+    dict(FDTD_HIP_NT="0"),
+]
+
+
+@pytest.mark.parametrize("backend", ["hip.float64", "hip.float32", "hip.float32c"])
+def test_launch_shape_overrides_leave_results_bit_identical(backend, monkeypatch):
+    """The launch-shape overrides -- row or flat tiles, x planes per
+    workgroup, chunk dealing, workgroup size, non-temporal accesses -- change
+    the schedule only: on the all-feature fixture scenario (every object
+    kind, point/line/plane sources, detectors, random initial fields) the
+    fields and detector traces equal the default launch's bit for bit."""
+    sc = scenarios.get_scenario("g2")
+
+    def run(env):
+        for k in LAUNCH_OVERRIDES:
+            monkeypatch.delenv(k, raising=False)
+        for k, v in env.items():
+            monkeypatch.setenv(k, v)
+        fdtd.set_backend(backend)
+        grid, comps = _build(sc)
+        scenarios.run_program(grid, sc)
+        torch.cuda.synchronize()
+        out = [_np(grid.full_precision("E") if backend == "hip.float32c" else grid.E),
+               _np(grid.full_precision("H") if backend == "hip.float32c" else grid.H)]
+        for c in comps:
+            if isinstance(c, fdtd.LineDetector):
+                out += [_np(c.trace("E")), _np(c.trace("H"))]
+        return out
+
+    ref = run({})
+    for env in LAUNCH_SHAPES:
+        got = run(env)
+        assert len(got) == len(ref)
+        for i, (a, b) in enumerate(zip(got, ref)):
+            assert np.array_equal(a, b), f"{backend} {env}: array {i} differs from the default launch"
+    _say(f"[gpu] {backend}: {len(LAUNCH_SHAPES)} launch shapes bit-identical to the default")
 
 
 @pytest.mark.parametrize("nz", [70, 100, 130])
