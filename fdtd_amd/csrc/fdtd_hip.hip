@@ -138,8 +138,10 @@ struct FdtdBatch {
   void* sims_H = nullptr;
   int64_t* first = nullptr;
   int32_t* item_sim = nullptr;
+  int32_t* rot = nullptr;
   int64_t* q = nullptr;
   int64_t total = 0;
+  bool cyclic = false;        // x-chunks dealt cyclically to the XCDs (BatchArgs)
   bool objs = false;          // some grid has objects: the E kernel variant with them
   // captured steps
   hipStream_t gstream = nullptr;
@@ -634,18 +636,42 @@ void batch_free_graph(FdtdBatch* b) {
   b->gsteps = 0;
 }
 
+// Whether a batch deals its x-chunks cyclically to the XCDs: once the fields
+// outgrow the caches (16.7M cells in all, as for one grid) and the grids are
+// not all of one shape. Contiguous runs of equal grids load the XCDs evenly
+// and keep each x-halo plane in the L2 of the XCD that read it; grids of
+// unequal shapes cost unequally per tile (CPML-heavy small grids beside large
+// ones), so contiguous runs of them leave some XCDs working while others
+// wait. FDTD_HIP_CYCLIC overrides.
+bool batch_cyclic(const std::vector<FdtdPlan*>& plans) {
+  int64_t cells = 0;
+  bool same = true;
+  for (const FdtdPlan* p : plans) {
+    cells += p->d.cs;
+    same = same && p->d.nx == plans[0]->d.nx && p->d.ny == plans[0]->d.ny && p->d.nz == plans[0]->d.nz;
+  }
+  bool cyc = cells >= (int64_t)1 << 24 && !same;
+  if (const char* env = getenv("FDTD_HIP_CYCLIC")) cyc = atoi(env) != 0;
+  return cyc;
+}
+
 // Per-grid kernel arguments of a batch for a launch shape: the E and H main
-// arguments of every plan and first[s], the first work item of grid s
-// (first[n] = the number of work items).
+// arguments of every plan, first[s], the first position of grid s in the
+// work list (first[n] = the number of positions), and rot[s], the XCD of grid
+// s's first chunk under cyclic dealing (the chunks of all grids in turn dealt
+// to XCD 0, 1, ..., 7, 0, ...; a grid spans whole rounds of 8 chunks).
 template <class FS, class MS>
-void batch_tables(const std::vector<FdtdPlan*>& plans, int block_threads, int cx, std::vector<MainArgs<FS, MS>>& sE,
-                  std::vector<MainArgs<FS, MS>>& sH, std::vector<int64_t>& first, bool& objs) {
+void batch_tables(const std::vector<FdtdPlan*>& plans, int block_threads, int cx, bool cyclic,
+                  std::vector<MainArgs<FS, MS>>& sE, std::vector<MainArgs<FS, MS>>& sH, std::vector<int64_t>& first,
+                  std::vector<int32_t>& rot, bool& objs) {
   using MA = MainArgs<FS, MS>;
   const size_t n = plans.size();
   sE.assign(n, MA{});
   sH.assign(n, MA{});
   first.assign(n + 1, 0);
+  rot.assign(n, 0);
   objs = false;
+  int64_t chunks = 0;  // chunks of the grids before this one
   for (size_t i = 0; i < n; ++i) {
     FdtdPlan* p = plans[i];
     MA aE = main_args<FS, MS>(p, false), aH = main_args<FS, MS>(p, true);
@@ -664,18 +690,22 @@ void batch_tables(const std::vector<FdtdPlan*>& plans, int block_threads, int cx
     objs = objs || aE.nobjs > 0;
     sE[i] = aE;
     sH[i] = aH;
-    const int64_t tiles = (int64_t)plane_tiles(p->d, aE.bz_shift, aE.by, aE.flat) * ((p->d.nx + aE.cx - 1) / aE.cx);
-    first[i + 1] = first[i] + tiles;
+    const int64_t per = plane_tiles(p->d, aE.bz_shift, aE.by, aE.flat), nch = (p->d.nx + aE.cx - 1) / aE.cx;
+    rot[i] = (int32_t)(chunks % 8);
+    const int64_t span = cyclic ? 8 * per * ((rot[i] + nch + 7) / 8) : per * nch;
+    chunks += nch;
+    first[i + 1] = first[i] + span;
   }
 }
 
 // Host image of a batch's device tables for a launch shape: the E and H
-// argument tables, first[], the time index slot and item_sim[], at the byte
-// offsets below (each 256-byte aligned).
+// argument tables, first[], rot[], the time index slot and item_sim[], at the
+// byte offsets below (each 256-byte aligned).
 struct BatchImage {
   std::vector<unsigned char> bytes;
-  size_t oE = 0, oH = 0, oF = 0, oQ = 0, oI = 0;
-  int64_t total = 0;  // work items
+  size_t oE = 0, oH = 0, oF = 0, oR = 0, oQ = 0, oI = 0;
+  int64_t total = 0;  // positions of the work list
+  bool cyclic = false;
   bool objs = false;
 };
 
@@ -685,19 +715,23 @@ int batch_image(const std::vector<FdtdPlan*>& plans, int block_threads, int cx, 
     using MA = MainArgs<FS, MS>;
     std::vector<MA> sE, sH;
     std::vector<int64_t> first;
-    batch_tables<FS, MS>(plans, block_threads, cx, sE, sH, first, img.objs);
+    std::vector<int32_t> rot;
+    img.cyclic = batch_cyclic(plans);
+    batch_tables<FS, MS>(plans, block_threads, cx, img.cyclic, sE, sH, first, rot, img.objs);
     img.total = first[n];
     if (img.total >= (int64_t)1 << 31) return fail(-1, "batch has too many work items");
     img.oE = 0;
     img.oH = align_up(n * sizeof(MA));
     img.oF = img.oH + align_up(n * sizeof(MA));
-    img.oQ = img.oF + align_up((n + 1) * sizeof(int64_t));
+    img.oR = img.oF + align_up((n + 1) * sizeof(int64_t));
+    img.oQ = img.oR + align_up(n * sizeof(int32_t));
     img.oI = img.oQ + 256;
     img.bytes.assign(img.oI + align_up(img.total * sizeof(int32_t)), 0);
     unsigned char* h = img.bytes.data();
     std::memcpy(h + img.oE, sE.data(), n * sizeof(MA));
     std::memcpy(h + img.oH, sH.data(), n * sizeof(MA));
     std::memcpy(h + img.oF, first.data(), (n + 1) * sizeof(int64_t));
+    std::memcpy(h + img.oR, rot.data(), n * sizeof(int32_t));
     int32_t* item_sim = reinterpret_cast<int32_t*>(h + img.oI);
     for (size_t i = 0; i < n; ++i)
       for (int64_t t = first[i]; t < first[i + 1]; ++t) item_sim[t] = (int32_t)i;
@@ -718,6 +752,7 @@ int batch_upload(FdtdBatch* b) {
   BatchImage img;
   if (int rc = batch_image(b->plans, b->block_threads, b->cx, img)) return rc;
   b->total = img.total;
+  b->cyclic = img.cyclic;
   b->objs = img.objs;
   FDTD_HIP(hipMalloc(&b->dev, img.bytes.size()));
   FDTD_HIP(hipMemcpy(b->dev, img.bytes.data(), img.bytes.size(), hipMemcpyHostToDevice));
@@ -725,6 +760,7 @@ int batch_upload(FdtdBatch* b) {
   b->sims_E = base + img.oE;
   b->sims_H = base + img.oH;
   b->first = reinterpret_cast<int64_t*>(base + img.oF);
+  b->rot = reinterpret_cast<int32_t*>(base + img.oR);
   b->q = reinterpret_cast<int64_t*>(base + img.oQ);
   b->item_sim = reinterpret_cast<int32_t*>(base + img.oI);
   return 0;
@@ -741,8 +777,10 @@ struct BatchTables {
   const void* sims_H;
   const int64_t* first;
   const int32_t* item_sim;
+  const int32_t* rot;
   int64_t total;
   int64_t* q;
+  bool cyclic;
 };
 
 // nsteps batched steps on stream st (the time index already set on the
@@ -751,7 +789,8 @@ int batch_launch_tables(const FdtdPlan* p0, const BatchTables& t, bool objs, int
                         int64_t nsteps, hipStream_t st) {
   const dim3 grid((unsigned)groups), block((unsigned)block_threads);
   return with_policies(p0, [&]<class FS, class MS, typename O>() -> int {
-    BatchArgs<FS, MS> aE{reinterpret_cast<const MainArgs<FS, MS>*>(t.sims_E), t.first, t.item_sim, t.total, t.q};
+    BatchArgs<FS, MS> aE{reinterpret_cast<const MainArgs<FS, MS>*>(t.sims_E), t.first, t.item_sim, t.rot, t.total, t.q,
+                         t.cyclic ? 1 : 0};
     BatchArgs<FS, MS> aH = aE;
     aH.sims = reinterpret_cast<const MainArgs<FS, MS>*>(t.sims_H);
     for (int64_t s = 0; s < nsteps; ++s) {
@@ -767,15 +806,21 @@ int batch_launch_tables(const FdtdPlan* p0, const BatchTables& t, bool objs, int
 }
 
 int batch_launch(FdtdBatch* b, int64_t nsteps, hipStream_t st) {
-  const BatchTables t{b->sims_E, b->sims_H, b->first, b->item_sim, b->total, b->q};
+  const BatchTables t{b->sims_E, b->sims_H, b->first, b->item_sim, b->rot, b->total, b->q, b->cyclic};
   return batch_launch_tables(b->plans[0], t, b->objs, batch_groups(b), b->block_threads, nsteps, st);
 }
 
 // Pointers to the tables of an image placed at device address base.
 BatchTables image_tables(const BatchImage& img, void* base) {
   char* c = reinterpret_cast<char*>(base);
-  return BatchTables{c + img.oE, c + img.oH, reinterpret_cast<const int64_t*>(c + img.oF),
-                     reinterpret_cast<const int32_t*>(c + img.oI), img.total, reinterpret_cast<int64_t*>(c + img.oQ)};
+  return BatchTables{c + img.oE,
+                     c + img.oH,
+                     reinterpret_cast<const int64_t*>(c + img.oF),
+                     reinterpret_cast<const int32_t*>(c + img.oI),
+                     reinterpret_cast<const int32_t*>(c + img.oR),
+                     img.total,
+                     reinterpret_cast<int64_t*>(c + img.oQ),
+                     img.cyclic};
 }
 
 // ----------------------------------------------- batch without allocation

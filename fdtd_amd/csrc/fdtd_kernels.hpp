@@ -830,48 +830,89 @@ __global__ void __launch_bounds__(1024) k_main_H(MainArgs<FS, MS> a) {
 }
 
 // ------------------------------------------------------------ batched kernels
-// Many independent grids stepped by one launch per half step. The work list
-// is every grid's tiles in turn (first[s] = first item of grid s). Workgroup
-// w takes items w', w' + G, w' + 2G, ... (G workgroups, w' = w dealt to the
-// XCDs so that each XCD gets a contiguous run of items: neighbouring tiles
-// share halo planes in its L2). With G equal to the number of items every
-// workgroup has one tile, as for a single grid; a smaller G makes the grid of
-// workgroups persistent. The time index lives in device memory so that a
-// captured graph of steps can be replayed: the E kernel reads it for the
-// folded point sources, the H kernel advances it.
+// Many independent grids stepped by one launch per half step over a work
+// list of positions (first[s] = first position of grid s, item_sim[u] = the
+// grid of position u). Workgroup w takes positions w', w' + G, w' + 2G, ...
+// (G workgroups). With G equal to the number of positions every workgroup has
+// one tile, as for a single grid; a smaller G makes the grid of workgroups
+// persistent. Two dealings to the 8 XCDs (workgroup w runs on XCD w mod 8):
+//  * contiguous (small batches): the positions are every grid's tiles in
+//    turn and w' = w dealt so that each XCD gets a contiguous run of them
+//    (neighbouring tiles share halo planes in its L2);
+//  * cyclic (batches whose fields outgrow the caches): w' = w, and the
+//    x-chunks of all grids, taken in turn, are dealt to the XCDs in rotation
+//    -- chunk g of that sequence to XCD g mod 8 (grid s starts at XCD rot[s])
+//    -- each XCD walking the tiles of its chunks in order. Every XCD then gets
+//    an eighth of every grid, so grids of unequal cost per tile (CPML-heavy
+//    small grids beside large ones) load the XCDs evenly, and at any moment
+//    the 8 XCDs work on neighbouring chunks, as a single grid's cyclic
+//    launch does. A grid spans whole rounds of 8 chunks; the positions of a
+//    round that fall before its first chunk or after its last are idle.
+// The time index lives in device memory so that a captured graph of steps
+// can be replayed: the E kernel reads it for the folded point sources, the H
+// kernel advances it.
 template <class FS, class MS>
 struct BatchArgs {
   const MainArgs<FS, MS>* sims;  // per grid, launch shape included
   const int64_t* first;          // nsim + 1 entries
-  const int32_t* item_sim;       // grid of each work item
-  int64_t total;                 // work items
+  const int32_t* item_sim;       // grid of each position
+  const int32_t* rot;            // cyclic dealing: XCD of each grid's first chunk
+  int64_t total;                 // positions (cyclic: a multiple of 8)
   int64_t* q;                    // time index of the next E half step
+  int32_t cyclic;
 };
+
+// Tile of grid a at position `local` of its span under cyclic dealing
+// (the grid's first chunk on XCD rot), or -1 for an idle position.
+template <class FS, class MS>
+__device__ __forceinline__ int64_t batch_cyclic_tile(const MainArgs<FS, MS>& a, unsigned local, int rot) {
+  unsigned per, nch;
+  chunk_counts(a, per, nch);
+  const unsigned x = local % 8, k = local / 8;
+  const int c = (int)((k / per) * 8 + x) - rot;
+  return c >= 0 && c < (int)nch ? (int64_t)c * per + k % per : -1;
+}
 
 template <class FS, class MS, typename O, bool OBJS, bool NT>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(EWaves<FS>::value)))
 k_batch_E(BatchArgs<FS, MS> b) {
   const int64_t q = *b.q;
-  for (int64_t it = xcd_tile(blockIdx.x, gridDim.x); it < b.total; it += gridDim.x) {
-    // the item and its grid are the same for the whole workgroup: say so,
-    // so that the grid's arguments are read with scalar loads (and, through
-    // the restrict reference of the tile body, not read again after the
-    // field stores)
+  for (int64_t it = b.cyclic ? (int64_t)blockIdx.x : (int64_t)xcd_tile(blockIdx.x, gridDim.x); it < b.total;
+       it += gridDim.x) {
+    // the position and its grid are the same for the whole workgroup: say
+    // so, so that the grid's arguments are read with scalar loads (and,
+    // through the restrict reference of the tile body, not read again after
+    // the field stores)
     const int64_t u = __builtin_amdgcn_readfirstlane((int)it);
     const int s = __builtin_amdgcn_readfirstlane(gp(b.item_sim)[u]);
-    main_E_tile<FS, MS, O, OBJS, NT>(b.sims[s], (unsigned)(u - b.first[s]), q);
+    int64_t t = u - b.first[s];
+    if (b.cyclic) {
+      t = batch_cyclic_tile(b.sims[s], (unsigned)t, b.rot[s]);
+      if (t < 0) continue;
+    }
+    main_E_tile<FS, MS, O, OBJS, NT>(b.sims[s], (unsigned)t, q);
   }
 }
 
-// H half step: the list walked from its end (the mirror of the E kernel's
-// order, as for one grid); one thread advances the time index (nothing else
-// in this kernel reads it, and the E kernel of this step has completed).
+// H half step: each grid's tiles in decreasing order (the mirror of the E
+// kernel's order, as for one grid; contiguous dealing walks the whole list
+// from its end); one thread advances the time index (nothing else in this
+// kernel reads it, and the E kernel of this step has completed).
 template <class FS, class MS, bool NT>
 __global__ void __launch_bounds__(1024) k_batch_H(BatchArgs<FS, MS> b) {
-  for (int64_t it = xcd_tile(blockIdx.x, gridDim.x); it < b.total; it += gridDim.x) {
-    const int64_t u = __builtin_amdgcn_readfirstlane((int)(b.total - 1 - it));
+  for (int64_t it = b.cyclic ? (int64_t)blockIdx.x : (int64_t)xcd_tile(blockIdx.x, gridDim.x); it < b.total;
+       it += gridDim.x) {
+    const int64_t u = __builtin_amdgcn_readfirstlane((int)(b.cyclic ? it : b.total - 1 - it));
     const int s = __builtin_amdgcn_readfirstlane(gp(b.item_sim)[u]);
-    main_H_tile<FS, MS, NT>(b.sims[s], (unsigned)(u - b.first[s]));
+    int64_t t = u - b.first[s];
+    if (b.cyclic) {
+      unsigned per, nch;
+      chunk_counts(b.sims[s], per, nch);
+      t = batch_cyclic_tile(b.sims[s], (unsigned)t, b.rot[s]);
+      if (t < 0) continue;
+      t = (int64_t)per * nch - 1 - t;
+    }
+    main_H_tile<FS, MS, NT>(b.sims[s], (unsigned)t);
   }
   // after the work: a branch at the top would keep the compiler from
   // treating the tile loop's values as uniform

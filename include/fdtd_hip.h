@@ -226,7 +226,7 @@ int fdtd_batch_run_graph(FdtdBatch* batch, int64_t q0, int64_t nsteps, int64_t s
  * until the process ends, instead of in memory it allocates. No device
  * memory is allocated or freed. The tables of a launch shape that need more
  * than FDTD_STATIC_BATCH_BYTES are refused (error -6); the 64-grid,
- * 132-million-cell batch needs about 2.7 MB at 256 threads and 1 plane.
+ * 132-million-cell batch needs about 2.2 MB at 256 threads and 1 plane.
  * The call copies the tables on `stream` and waits for that stream (work
  * queued on it before, then the copy), so it is not for graph capture.
  * Every call on a GPU shares that GPU's array and leaves its tables there:

@@ -25,8 +25,9 @@ the device library gets. Prints one JSON line per case: {"case", "ok",
                       against the guarded form, bit for bit
   batch:<seed>        grids of uneven shapes stepped by fdtd_batch_run
                       (direct launches, graph replays and a persistent
-                      launch) and by fdtd_batch_run_static, bit for bit
-                      against the same grids stepped alone
+                      launch) and by fdtd_batch_run_static, with the x-chunks
+                      dealt to the XCDs contiguously and cyclically, bit for
+                      bit against the same grids stepped alone
 """
 
 import ctypes
@@ -312,13 +313,20 @@ def case_batch(seed):
         g.run(steps, progress_bar=False)
     want = _state(alone)
     errs = []
-    for mode, launch in (("direct", (256, 1, 0)), ("graph", (256, 2, 0)), ("persistent", (512, 1, 3)),
-                         ("static", (256, 2, 0))):
-        grids = _batch_grids(seed)
-        errs += _run_batch(mode, launch, grids, steps)
-        for i, (x, y) in enumerate(zip(_state(grids), want)):
-            if not torch.equal(x, y):
-                errs.append(f"{mode} {launch}: state tensor {i} differs")
+    runs = [(cyc, mode, launch) for cyc in ("0", "1")
+            for mode, launch in (("direct", (256, 1, 0)), ("graph", (256, 2, 0)), ("persistent", (512, 1, 3)),
+                                 ("persistent", (256, 3, 16)), ("static", (256, 2, 0)))]
+    try:
+        for cyc, mode, launch in runs:
+            # small batches deal contiguously unless told otherwise
+            os.environ["FDTD_HIP_CYCLIC"] = cyc
+            grids = _batch_grids(seed)
+            errs += _run_batch(mode, launch, grids, steps)
+            for i, (x, y) in enumerate(zip(_state(grids), want)):
+                if not torch.equal(x, y):
+                    errs.append(f"cyclic={cyc} {mode} {launch}: state tensor {i} differs")
+    finally:
+        os.environ.pop("FDTD_HIP_CYCLIC", None)
     return errs
 
 

@@ -51,20 +51,28 @@ def _fields(grids):
     return [(g.full_precision("E").cpu().numpy(), g.full_precision("H").cpu().numpy()) for g in grids]
 
 
+@pytest.mark.parametrize("dealing", ["contiguous", "cyclic"])
 @pytest.mark.parametrize("backend", ["hip.float64", "hip.float32", "hip.float32c"])
-def test_batch_matches_grids_stepped_alone(backend):
+def test_batch_matches_grids_stepped_alone(backend, dealing, monkeypatch):
+    """Small batches deal their tiles to the XCDs contiguously by default;
+    FDTD_HIP_CYCLIC=1 forces the cyclic dealing of large batches (idle
+    positions at both ends of every grid's span), which must not change a
+    bit."""
     fdtd.set_backend(backend)
     steps = 37
     ref = make_grids(3)
     for g in ref:
         g.run(steps, progress_bar=False)
     want = _fields(ref)
+    if dealing == "cyclic":
+        monkeypatch.setenv("FDTD_HIP_CYCLIC", "1")
     for graph, shape in ((False, (256, 1, 0)), (True, (512, 2, 64)), (True, (256, 3, 1000))):
         grids = make_grids(3)
         batch = GridBatch(grids, graph_steps=10)
         batch.set_launch(*shape)
-        _say(f"[gpu] batch {backend} graph={graph} launch={shape}")
+        _say(f"[gpu] batch {backend} {dealing} graph={graph} launch={shape}")
         batch.run(steps, graph=graph)
+        _say(f"[gpu]   work list positions: {batch._batch.work_items()}")
         got = _fields(grids)
         for i, ((eg, hg), (ew, hw)) in enumerate(zip(got, want)):
             assert np.array_equal(eg, ew) and np.array_equal(hg, hw), f"grid {i} differs ({backend}, {shape})"
