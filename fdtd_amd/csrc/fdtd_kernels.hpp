@@ -730,14 +730,17 @@ __device__ __forceinline__ int64_t launch_tile(const MainArgs<FS, MS>& a, unsign
 // VGPRs (4 waves per SIMD) for the anisotropic 3x3 path. Under a floor of 6
 // the double-precision ones fit in ~76 VGPRs without spilling, the float32
 // ones under a floor of 8 in ~46 (a floor of 8 makes the double ones spill);
-// complex128 values need twice the registers and keep a floor of 4.
-template <class FS>
+// complex128 values need twice the registers and keep a floor of 4. The
+// compensated kernels without objects fit 8 waves in 62 VGPRs (66 and 7
+// waves unbounded; 0.6 % faster per step at 256^3).
+template <class FS, bool OBJS = true>
 struct EWaves {
-  static constexpr int value = sizeof(typename FS::R) == 4 ? 8 : (sizeof(typename FS::R) == 8 ? 6 : 4);
+  static constexpr int value =
+      FS::kRecords && !OBJS ? 8 : (sizeof(typename FS::R) == 4 ? 8 : (sizeof(typename FS::R) == 8 ? 6 : 4));
 };
 
 template <class FS, class MS, typename O, bool OBJS, bool NT, bool REC>
-__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(EWaves<FS>::value)))
+__global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(EWaves<FS, OBJS>::value)))
 k_main_E(MainArgs<FS, MS> a) {
   unsigned n;
   const int64_t t = launch_tile(a, n);
