@@ -793,12 +793,20 @@ int batch_launch_tables(const FdtdPlan* p0, const BatchTables& t, bool objs, int
                          t.cyclic ? 1 : 0};
     BatchArgs<FS, MS> aH = aE;
     aH.sims = reinterpret_cast<const MainArgs<FS, MS>*>(t.sims_H);
+    const bool loop = groups < t.total;  // a persistent grid of workgroups
     for (int64_t s = 0; s < nsteps; ++s) {
-      if (objs)
-        hipLaunchKernelGGL((k_batch_E<FS, MS, O, true, true>), grid, block, 0, st, aE);
+      if (objs && loop)
+        hipLaunchKernelGGL((k_batch_E<FS, MS, O, true, true, true>), grid, block, 0, st, aE);
+      else if (objs)
+        hipLaunchKernelGGL((k_batch_E<FS, MS, O, true, true, false>), grid, block, 0, st, aE);
+      else if (loop)
+        hipLaunchKernelGGL((k_batch_E<FS, MS, O, false, true, true>), grid, block, 0, st, aE);
       else
-        hipLaunchKernelGGL((k_batch_E<FS, MS, O, false, true>), grid, block, 0, st, aE);
-      hipLaunchKernelGGL((k_batch_H<FS, MS, true>), grid, block, 0, st, aH);
+        hipLaunchKernelGGL((k_batch_E<FS, MS, O, false, true, false>), grid, block, 0, st, aE);
+      if (loop)
+        hipLaunchKernelGGL((k_batch_H<FS, MS, true, true>), grid, block, 0, st, aH);
+      else
+        hipLaunchKernelGGL((k_batch_H<FS, MS, true, false>), grid, block, 0, st, aH);
     }
     FDTD_HIP(hipGetLastError());
     return 0;

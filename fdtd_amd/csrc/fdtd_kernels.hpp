@@ -876,46 +876,68 @@ __device__ __forceinline__ int64_t batch_cyclic_tile(const MainArgs<FS, MS>& a, 
   return c >= 0 && c < (int)nch ? (int64_t)c * per + k % per : -1;
 }
 
+// One position of a batched E launch. The position and its grid are the
+// same for the whole workgroup: say so, so that the grid's arguments are read
+// with scalar loads (and, through the restrict reference of the tile body,
+// not read again after the field stores). The launch arguments come by value:
+// taken by reference, the pointers read from them lost their global address
+// space, and the face tables were read with vector (flat) loads, each cell's
+// face loop waiting on memory (twice the time per launch).
 template <class FS, class MS, typename O, bool OBJS, bool NT>
+__device__ __forceinline__ void batch_E_at(BatchArgs<FS, MS> b, int64_t it, int64_t q) {
+  const int64_t u = __builtin_amdgcn_readfirstlane((int)it);
+  const int s = __builtin_amdgcn_readfirstlane(gp(b.item_sim)[u]);
+  int64_t t = u - b.first[s];
+  if (b.cyclic) {
+    t = batch_cyclic_tile(b.sims[s], (unsigned)t, b.rot[s]);
+    if (t < 0) return;
+  }
+  main_E_tile<FS, MS, O, OBJS, NT>(b.sims[s], (unsigned)t, q);
+}
+
+// One position of a batched H launch: each grid's tiles in decreasing order
+// (the mirror of the E kernel's order, as for one grid; contiguous dealing
+// walks the whole list from its end).
+template <class FS, class MS, bool NT>
+__device__ __forceinline__ void batch_H_at(BatchArgs<FS, MS> b, int64_t it) {
+  const int64_t u = __builtin_amdgcn_readfirstlane((int)(b.cyclic ? it : b.total - 1 - it));
+  const int s = __builtin_amdgcn_readfirstlane(gp(b.item_sim)[u]);
+  int64_t t = u - b.first[s];
+  if (b.cyclic) {
+    unsigned per, nch;
+    chunk_counts(b.sims[s], per, nch);
+    t = batch_cyclic_tile(b.sims[s], (unsigned)t, b.rot[s]);
+    if (t < 0) return;
+    t = (int64_t)per * nch - 1 - t;
+  }
+  main_H_tile<FS, MS, NT>(b.sims[s], (unsigned)t);
+}
+
+// LOOP: a persistent grid of workgroups walking the list; otherwise one
+// position per workgroup, with no loop around the tile body (the loop's
+// live values made the compiler spill scalar registers to vector lanes,
+// ~50 more vector instructions per wavefront).
+template <class FS, class MS, typename O, bool OBJS, bool NT, bool LOOP>
 __global__ void __launch_bounds__(1024) __attribute__((amdgpu_waves_per_eu(EWaves<FS>::value)))
 k_batch_E(BatchArgs<FS, MS> b) {
   const int64_t q = *b.q;
-  for (int64_t it = b.cyclic ? (int64_t)blockIdx.x : (int64_t)xcd_tile(blockIdx.x, gridDim.x); it < b.total;
-       it += gridDim.x) {
-    // the position and its grid are the same for the whole workgroup: say
-    // so, so that the grid's arguments are read with scalar loads (and,
-    // through the restrict reference of the tile body, not read again after
-    // the field stores)
-    const int64_t u = __builtin_amdgcn_readfirstlane((int)it);
-    const int s = __builtin_amdgcn_readfirstlane(gp(b.item_sim)[u]);
-    int64_t t = u - b.first[s];
-    if (b.cyclic) {
-      t = batch_cyclic_tile(b.sims[s], (unsigned)t, b.rot[s]);
-      if (t < 0) continue;
-    }
-    main_E_tile<FS, MS, O, OBJS, NT>(b.sims[s], (unsigned)t, q);
+  const int64_t it0 = b.cyclic ? (int64_t)blockIdx.x : (int64_t)xcd_tile(blockIdx.x, gridDim.x);
+  if constexpr (LOOP) {
+    for (int64_t it = it0; it < b.total; it += gridDim.x) batch_E_at<FS, MS, O, OBJS, NT>(b, it, q);
+  } else if (it0 < b.total) {
+    batch_E_at<FS, MS, O, OBJS, NT>(b, it0, q);
   }
 }
 
-// H half step: each grid's tiles in decreasing order (the mirror of the E
-// kernel's order, as for one grid; contiguous dealing walks the whole list
-// from its end); one thread advances the time index (nothing else in this
+// H half step; one thread advances the time index (nothing else in this
 // kernel reads it, and the E kernel of this step has completed).
-template <class FS, class MS, bool NT>
+template <class FS, class MS, bool NT, bool LOOP>
 __global__ void __launch_bounds__(1024) k_batch_H(BatchArgs<FS, MS> b) {
-  for (int64_t it = b.cyclic ? (int64_t)blockIdx.x : (int64_t)xcd_tile(blockIdx.x, gridDim.x); it < b.total;
-       it += gridDim.x) {
-    const int64_t u = __builtin_amdgcn_readfirstlane((int)(b.cyclic ? it : b.total - 1 - it));
-    const int s = __builtin_amdgcn_readfirstlane(gp(b.item_sim)[u]);
-    int64_t t = u - b.first[s];
-    if (b.cyclic) {
-      unsigned per, nch;
-      chunk_counts(b.sims[s], per, nch);
-      t = batch_cyclic_tile(b.sims[s], (unsigned)t, b.rot[s]);
-      if (t < 0) continue;
-      t = (int64_t)per * nch - 1 - t;
-    }
-    main_H_tile<FS, MS, NT>(b.sims[s], (unsigned)t);
+  const int64_t it0 = b.cyclic ? (int64_t)blockIdx.x : (int64_t)xcd_tile(blockIdx.x, gridDim.x);
+  if constexpr (LOOP) {
+    for (int64_t it = it0; it < b.total; it += gridDim.x) batch_H_at<FS, MS, NT>(b, it);
+  } else if (it0 < b.total) {
+    batch_H_at<FS, MS, NT>(b, it0);
   }
   // after the work: a branch at the top would keep the compiler from
   // treating the tile loop's values as uniform
