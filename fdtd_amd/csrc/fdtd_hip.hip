@@ -636,21 +636,15 @@ void batch_free_graph(FdtdBatch* b) {
   b->gsteps = 0;
 }
 
-// Whether a batch deals its x-chunks cyclically to the XCDs: once the fields
-// outgrow the caches (16.7M cells in all, as for one grid) and the grids are
-// not all of one shape. Contiguous runs of equal grids load the XCDs evenly
-// and keep each x-halo plane in the L2 of the XCD that read it; grids of
-// unequal shapes cost unequally per tile (CPML-heavy small grids beside large
-// ones), so contiguous runs of them leave some XCDs working while others
-// wait. FDTD_HIP_CYCLIC overrides.
+// Whether a batch deals its x-chunks cyclically to the XCDs: as for one grid,
+// once the fields outgrow the caches (16.7M cells in all). Contiguous runs of
+// work items leave some XCDs working while others wait: the grids' CPML slabs
+// and their unequal shapes make tiles cost unequally. FDTD_HIP_CYCLIC
+// overrides.
 bool batch_cyclic(const std::vector<FdtdPlan*>& plans) {
   int64_t cells = 0;
-  bool same = true;
-  for (const FdtdPlan* p : plans) {
-    cells += p->d.cs;
-    same = same && p->d.nx == plans[0]->d.nx && p->d.ny == plans[0]->d.ny && p->d.nz == plans[0]->d.nz;
-  }
-  bool cyc = cells >= (int64_t)1 << 24 && !same;
+  for (const FdtdPlan* p : plans) cells += p->d.cs;
+  bool cyc = cells >= (int64_t)1 << 24;
   if (const char* env = getenv("FDTD_HIP_CYCLIC")) cyc = atoi(env) != 0;
   return cyc;
 }
