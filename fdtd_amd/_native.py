@@ -28,6 +28,7 @@ _SIGNATURES = {
     "fdtd_last_error": ([], ctypes.c_char_p),
     "fdtd_plan_create": ([_c_int, _c_int, _c_int, _c_int, _c_dbl, _vp, _vp, _vp, _vp, ctypes.POINTER(_vp)], _c_int),
     "fdtd_plan_destroy": ([_vp], _c_int),
+    "fdtd_plan_set_pitch": ([_vp, _c_i64], _c_int),
     "fdtd_plan_set_compensation": ([_vp, _vp, _vp], _c_int),
     "fdtd_plan_set_material_palette": ([_vp, _vp, _vp, _vp, _vp], _c_int),
     "fdtd_plan_add_pml": ([_vp, _c_int, _c_int, _c_int, _pdbl, _pdbl, _pdbl, _pdbl, _vp, _vp], _c_int),
@@ -138,7 +139,7 @@ class Plan:
     """Owner of one FdtdPlan*; every device buffer it points at is a torch
     tensor that the caller keeps alive for the plan's lifetime."""
 
-    def __init__(self, dtype_code, shape, courant, E, H, inv_eps, inv_mu):
+    def __init__(self, dtype_code, shape, courant, E, H, inv_eps, inv_mu, pitch=None):
         L = lib()
         self._L = L
         h = _vp()
@@ -147,6 +148,8 @@ class Plan:
                                  ctypes.byref(h)))
         self._h = h
         self._keep = [E, H, inv_eps, inv_mu]
+        if pitch is not None and pitch != shape[1] * shape[2]:
+            check(L.fdtd_plan_set_pitch(h, int(pitch)))
 
     def __del__(self):
         h = getattr(self, "_h", None)

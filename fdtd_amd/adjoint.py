@@ -189,7 +189,13 @@ class IntensityGradient:
             else:
                 gcoef[sl] = gcoef[sl] + S * inv
         dt = g.dtype
-        return gcoef.to(dt).contiguous(), (dcoef.to(dt).contiguous() if dcoef is not None else None)
+
+        def pitched(t):  # the fields' (pitched) layout, which the kernels index
+            out = g._alloc_soa(dt)
+            out.copy_(t)
+            return out
+
+        return pitched(gcoef), (pitched(dcoef) if dcoef is not None else None)
 
     def _plan(self):
         """A plan of the grid's boundaries, objects and sources (no detectors)
@@ -203,7 +209,8 @@ class IntensityGradient:
         g = self.grid
         _require_device(g._E)
         code = _native.F64 if g.dtype == torch.float64 else _native.F32
-        plan = _native.Plan(code, (g.Nx, g.Ny, g.Nz), g.courant_number, g._E, g._H, g._inv_eps, g._inv_mu)
+        plan = _native.Plan(code, (g.Nx, g.Ny, g.Nz), g.courant_number, g._E, g._H, g._inv_eps, g._inv_mu,
+                            pitch=g._px)
         for comp in list(g.boundaries) + list(g.objects) + list(g.sources):
             comp._add_to_plan(plan)
         if g.sources:
@@ -355,8 +362,8 @@ class IntensityGradient:
         else:
             K, nbuf = max(1, math.isqrt(T - 1) + 1), 1
         gcoef, dcoef = self._coefficients()
-        lamE = torch.zeros_like(g._E)
-        lamH = torch.zeros_like(g._H)
+        lamE = g._alloc_soa(g._E.dtype)  # the fields' (pitched) layout
+        lamH = g._alloc_soa(g._H.dtype)
         psi = torch.zeros(_native.Adjoint.psi_size(plan), dtype=dt, device=dev)
         grad = torch.zeros((3, self.nd), dtype=torch.float64, device=dev)
         adj = _native.Adjoint(plan, gcoef, dcoef, lamE, lamH, psi, self.design_box, self.box, grad)

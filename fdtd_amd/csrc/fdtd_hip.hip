@@ -174,7 +174,17 @@ int check_plan(const FdtdPlan* p) {
   return 0;
 }
 
-int cell_ok(const FdtdPlan* p, int64_t c) { return c >= 0 && c < p->d.cs; }
+int cell_ok(const FdtdPlan* p, int64_t c) {
+  return c >= 0 && c < p->d.cs && c % p->d.px < (int64_t)p->d.ny * p->d.nz;  // not in a plane's padding
+}
+
+// cells of the grid (without the planes' padding)
+int64_t grid_cells(const FdtdPlan* p) { return (int64_t)p->d.nx * p->d.ny * p->d.nz; }
+
+// cells of one plane normal to `axis`
+int64_t plane_cells(const FdtdPlan* p, int axis) {
+  return grid_cells(p) / (axis == 0 ? p->d.nx : (axis == 1 ? p->d.ny : p->d.nz));
+}
 
 bool is_complex(const FdtdPlan* p) { return p->dtype == FDTD_C128 || p->dtype == FDTD_C64; }
 
@@ -331,7 +341,7 @@ int launch_half(FdtdPlan* p, bool is_H, int64_t q, hipStream_t st) {
       const int axis = b.index;
       const int n_axis = axis == 0 ? p->d.nx : (axis == 1 ? p->d.ny : p->d.nz);
       if (n_axis <= 1) continue;  // plane 0 is plane -1: the copy is the identity
-      const int64_t n_plane = p->d.cs / n_axis;
+      const int64_t n_plane = plane_cells(p, axis);
       hipLaunchKernelGGL(k_periodic<FS>, dim3((unsigned)((n_plane + 255) / 256)), dim3(256), 0, st, p->d, a.F,
                          axis, is_H ? 0 : 1);
     } else {
@@ -484,7 +494,7 @@ void adj_periodic(const FdtdPlan* p, T* lam, bool is_H, hipStream_t st) {
     const int axis = it->index;
     const int n_axis = axis == 0 ? p->d.nx : (axis == 1 ? p->d.ny : p->d.nz);
     if (n_axis <= 1) continue;  // the forward copy is the identity
-    const int64_t n_plane = p->d.cs / n_axis;
+    const int64_t n_plane = plane_cells(p, axis);
     hipLaunchKernelGGL(k_adj_periodic<T>, dim3((unsigned)((n_plane + 255) / 256)), dim3(256), 0, st, lam, p->d, axis,
                        is_H ? 0 : 1);
   }
@@ -643,7 +653,7 @@ void batch_free_graph(FdtdBatch* b) {
 // overrides.
 bool batch_cyclic(const std::vector<FdtdPlan*>& plans) {
   int64_t cells = 0;
-  for (const FdtdPlan* p : plans) cells += p->d.cs;
+  for (const FdtdPlan* p : plans) cells += grid_cells(p);
   bool cyc = cells >= (int64_t)1 << 24;
   if (const char* env = getenv("FDTD_HIP_CYCLIC")) cyc = atoi(env) != 0;
   return cyc;
@@ -889,7 +899,8 @@ int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number, v
   p->d.nx = nx;
   p->d.ny = ny;
   p->d.nz = nz;
-  p->d.cs = (int64_t)nx * ny * nz;
+  p->d.px = (int64_t)ny * nz;
+  p->d.cs = (int64_t)nx * p->d.px;
   p->sc = courant_number;
   p->E = E;
   p->H = H;
@@ -907,6 +918,18 @@ int fdtd_plan_destroy(FdtdPlan* plan) {
   if (!plan) return 0;
   free_tables(plan);
   delete plan;
+  return 0;
+}
+
+int fdtd_plan_set_pitch(FdtdPlan* p, int64_t px) {
+  if (!p) return fail(-1, "null plan");
+  if (px < (int64_t)p->d.ny * p->d.nz) return fail(-1, "the x-plane pitch must be at least ny*nz");
+  if (!p->faces.empty() || !p->objs.empty() || !p->srcs.empty() || !p->dets.empty() || !p->curs.empty() ||
+      !p->order.empty() || p->E_lo || p->eps_id)
+    return fail(-1, "set the pitch before anything is added to the plan");
+  p->d.px = px;
+  p->d.cs = (int64_t)p->d.nx * px;
+  p->finalized = false;
   return 0;
 }
 
@@ -1145,7 +1168,7 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     if (const char* env = getenv("FDTD_HIP_CX")) cx = atoi(env);
     if (const char* env = getenv("FDTD_HIP_NT")) p->nontemporal = atoi(env) != 0;
     // cyclic dealing pays once the fields outgrow the caches (16.7M cells = 256^3)
-    p->cyclic = p->d.cs >= (int64_t)1 << 24;
+    p->cyclic = grid_cells(p) >= (int64_t)1 << 24;
     if (const char* env = getenv("FDTD_HIP_CYCLIC")) p->cyclic = atoi(env) != 0;
     p->cx = (int)std::max<int64_t>(1, std::min<int64_t>(cx, p->d.nx));
   }
@@ -1215,7 +1238,7 @@ int fdtd_plan_finalize(FdtdPlan* p) {
     fd.cH = dcoef + 3 * f.t;
     fd.psiE = f.psiE;
     fd.psiH = f.psiH;
-    fd.slab = p->d.cs / (f.axis == 0 ? p->d.nx : (f.axis == 1 ? p->d.ny : p->d.nz)) * f.t;
+    fd.slab = plane_cells(p, f.axis) * f.t;
     p->h_faces.push_back(fd);
   }
   // fused faces first (registration order), then the staged ones
@@ -1352,7 +1375,7 @@ int fdtd_adjoint_psi_size(const FdtdPlan* p, int64_t* elements) {
   if (!p || !elements) return fail(-1, "null argument");
   int64_t n = 0;
   for (const FaceHost& f : p->faces)
-    n += 8 * (p->d.cs / (f.axis == 0 ? p->d.nx : (f.axis == 1 ? p->d.ny : p->d.nz)) * f.t);
+    n += 8 * (plane_cells(p, f.axis) * f.t);
   *elements = n;
   return 0;
 }

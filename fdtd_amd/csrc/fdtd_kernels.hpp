@@ -97,10 +97,19 @@ struct DetDev {
   int64_t slot0E, slot0H;  // slot written at half-step counter c is c - slot0
 };
 
+// Cell (i, j, k) of a component is at i*px + j*nz + k: z fastest, x planes
+// `px` cells apart (px >= ny*nz; the host rounds a plane up so that every x
+// plane starts on a cache line, and the cells past ny*nz are never read or
+// written), components cs = nx*px apart.
 struct Dims {
   int32_t nx, ny, nz, pad;
-  int64_t cs;  // component stride nx*ny*nz
+  int64_t cs;  // component stride nx*px
+  int64_t px;  // x-plane pitch
 };
+
+__host__ __device__ __forceinline__ int64_t cell_of(const Dims& d, int64_t i, int64_t j, int64_t k) {
+  return i * d.px + j * d.nz + k;
+}
 
 // ------------------------------------------------------- load/store helpers
 // Loads and stores of the read-once streams (the updated field and the
@@ -542,7 +551,7 @@ __device__ __forceinline__ int64_t box_index(const Box& b, int i, int j, int k) 
 __device__ __forceinline__ int64_t box_cell(const Box& b, const Dims& d, int64_t p) {
   const int64_t nz = b.z1 - b.z0, ny = b.y1 - b.y0;
   const int64_t k = b.z0 + p % nz, j = b.y0 + (p / nz) % ny, i = b.x0 + p / (nz * ny);
-  return (i * d.ny + j) * d.nz + k;
+  return cell_of(d, i, j, k);
 }
 
 // ------------------------------------------------------------ main kernels
@@ -634,7 +643,7 @@ __device__ __forceinline__ void main_E_tile(const MainArgs<FS, MS>& __restrict__
   int j, k, i0, i1;
   bool act;
   tile_column(a, tile, j, k, i0, i1, act);
-  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz;
+  const int64_t sy = d.nz, sx = d.px;
   const int64_t zm = k > 0 ? 1 : 0, ym = j > 0 ? sy : 0;
   const T zero = T(0);
   int64_t cell = (int64_t)i0 * sx + (int64_t)j * sy + k;
@@ -758,7 +767,7 @@ __device__ __forceinline__ void main_H_tile(const MainArgs<FS, MS>& __restrict__
   int j, k, i0, i1;
   bool act;
   tile_column(a, tile, j, k, i0, i1, act);
-  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz;
+  const int64_t sy = d.nz, sx = d.px;
   const int64_t zp = k < d.nz - 1 ? 1 : 0, yp = j < d.ny - 1 ? sy : 0;
   const T zero = T(0);
   int64_t cell = (int64_t)(i1 - 1) * sx + (int64_t)j * sy + k;
@@ -966,8 +975,8 @@ __global__ void __launch_bounds__(256) k_face(Dims d, typename FS::R sc, FS F, F
   const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
   const int g = fd.high ? n - fd.t + l : l;
   const int i = AX == 0 ? g : (int)li, j = AX == 1 ? g : (int)lj, k = AX == 2 ? g : (int)lk;
-  const int64_t cell = ((int64_t)i * d.ny + j) * d.nz + k;
-  const int64_t st = AX == 0 ? (int64_t)d.ny * d.nz : (AX == 1 ? (int64_t)d.nz : 1);
+  const int64_t cell = cell_of(d, i, j, k);
+  const int64_t st = AX == 0 ? d.px : (AX == 1 ? (int64_t)d.nz : 1);
   // the other field's components differenced by this face: 3-AX-C0, 3-AX-C1
   constexpr int S0 = 3 - AX - (AX == 0 ? 1 : 0), S1 = 3 - AX - (AX == 2 ? 1 : 2);
   const int64_t o0 = S0 * d.cs + cell, o1 = S1 * d.cs + cell;
@@ -1001,20 +1010,20 @@ __global__ void __launch_bounds__(256) k_periodic(Dims d, FS F, int axis, int to
   if (axis == 0) {
     n_plane = (int64_t)d.ny * d.nz;
     if (p >= n_plane) return;
-    const int64_t last = (int64_t)(d.nx - 1) * n_plane + p;
+    const int64_t last = (int64_t)(d.nx - 1) * d.px + p;
     src = to_low ? last : p;
     dst = to_low ? p : last;
   } else if (axis == 1) {
     n_plane = (int64_t)d.nx * d.nz;
     if (p >= n_plane) return;
     const int64_t i = p / d.nz, k = p % d.nz;
-    const int64_t first = (i * d.ny) * d.nz + k, last = (i * d.ny + d.ny - 1) * d.nz + k;
+    const int64_t first = cell_of(d, i, 0, k), last = cell_of(d, i, d.ny - 1, k);
     src = to_low ? last : first;
     dst = to_low ? first : last;
   } else {
     n_plane = (int64_t)d.nx * d.ny;
     if (p >= n_plane) return;
-    const int64_t first = p * d.nz, last = p * d.nz + d.nz - 1;
+    const int64_t first = cell_of(d, p / d.ny, p % d.ny, 0), last = first + d.nz - 1;
     src = to_low ? last : first;
     dst = to_low ? first : last;
   }
@@ -1097,7 +1106,7 @@ __global__ void __launch_bounds__(256) k_tail(TailArgs<FS> a) {
           x = x < 0 ? x + a.d.nx : x;
           y = y < 0 ? y + a.d.ny : y;
           z = z < 0 ? z + a.d.nz : z;
-          return a.F.get(c * a.d.cs + ((int64_t)x * a.d.ny + y) * a.d.nz + z);
+          return a.F.get(c * a.d.cs + cell_of(a.d, x, y, z));
         };
         T v1 = (H(0, px, py - 1, pz) - H(0, px, py, pz)) * dx;
         T v2 = (H(1, px, py, pz) - H(1, px - 1, py, pz)) * dx;
@@ -1283,7 +1292,7 @@ __device__ __forceinline__ void adj_faces_E(const AdjArgs<T>& a, int t_lo, int t
   const int64_t cs = d.cs;
   const int g = AX == 0 ? i : (AX == 1 ? j : k);
   const int n = AX == 0 ? d.nx : (AX == 1 ? d.ny : d.nz);
-  const int64_t st = AX == 0 ? (int64_t)d.ny * d.nz : (AX == 1 ? (int64_t)d.nz : 1);
+  const int64_t st = AX == 0 ? d.px : (AX == 1 ? (int64_t)d.nz : 1);
   const T sc = a.sc;
   auto K = [&](int c, int64_t x) { return sc * a.inv_eps[c * cs + x]; };
 #pragma nounroll
@@ -1358,7 +1367,7 @@ __global__ void __launch_bounds__(1024) k_adj_H(AdjArgs<T> a) {
   const bool act = FULL || (k0 < d.nz && j0 < d.ny);
   const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
   const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
-  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const int64_t sy = d.nz, sx = d.px, cs = d.cs;
   const int64_t zm = k > 0 ? 1 : 0, ym = j > 0 ? sy : 0;
   const T zero = T(0), sc = a.sc;
   auto Y = [&](int c, int64_t x) { return (sc * a.inv_mu[c * cs + x]) * a.lamH[c * cs + x]; };
@@ -1451,9 +1460,11 @@ __global__ void __launch_bounds__(256) k_adj_Y(AdjArgs<T> a) {
   const int64_t cs = d.cs;
   const int64_t cell = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (cell >= cs) return;
+  const int64_t r = cell % d.px;  // cell of the x plane; past ny*nz: the plane's padding
+  if (r >= (int64_t)d.ny * d.nz) return;
   const T l0 = a.lamE[cell], l1 = a.lamE[cs + cell], l2 = a.lamE[2 * cs + cell];
   T y0 = a.gcoef[cell] * l0, y1 = a.gcoef[cs + cell] * l1, y2 = a.gcoef[2 * cs + cell] * l2;
-  const int k = (int)(cell % d.nz), j = (int)((cell / d.nz) % d.ny), i = (int)(cell / ((int64_t)d.ny * d.nz));
+  const int k = (int)(r % d.nz), j = (int)(r / d.nz), i = (int)(cell / d.px);
   for (int o = 0; o < a.nobjs; ++o) {
     const ObjDev& ob = a.objs[o];
     if (ob.kind != FDTD_OBJ_ANISOTROPIC) continue;
@@ -1481,16 +1492,16 @@ __global__ void __launch_bounds__(256) k_adj_periodic(T* lam, Dims d, int axis, 
     const int64_t n_plane = (int64_t)d.ny * d.nz;
     if (p >= n_plane) return;
     lo_cell = p;
-    hi_cell = (int64_t)(d.nx - 1) * n_plane + p;
+    hi_cell = (int64_t)(d.nx - 1) * d.px + p;
   } else if (axis == 1) {
     if (p >= (int64_t)d.nx * d.nz) return;
     const int64_t i = p / d.nz, k = p % d.nz;
-    lo_cell = (i * d.ny) * d.nz + k;
-    hi_cell = (i * d.ny + d.ny - 1) * d.nz + k;
+    lo_cell = cell_of(d, i, 0, k);
+    hi_cell = cell_of(d, i, d.ny - 1, k);
   } else {
     if (p >= (int64_t)d.nx * d.ny) return;
-    lo_cell = p * d.nz;
-    hi_cell = p * d.nz + d.nz - 1;
+    lo_cell = cell_of(d, p / d.ny, p % d.ny, 0);
+    hi_cell = lo_cell + d.nz - 1;
   }
   const int64_t dst = to_low ? lo_cell : hi_cell, src = to_low ? hi_cell : lo_cell;
 #pragma unroll
@@ -1514,7 +1525,7 @@ __global__ void __launch_bounds__(1024) k_adj_E(AdjArgs<T> a) {
   const bool act = k0 < d.nz && j0 < d.ny;
   const int k = min(k0, d.nz - 1), j = min(j0, d.ny - 1);
   const int i0 = tx * a.cx, i1 = min(i0 + a.cx, d.nx);
-  const int64_t sy = d.nz, sx = (int64_t)d.ny * d.nz, cs = d.cs;
+  const int64_t sy = d.nz, sx = d.px, cs = d.cs;
   const int64_t zp = k < d.nz - 1 ? 1 : 0, yp = j < d.ny - 1 ? sy : 0;
   const T zero = T(0), sc = a.sc;
   auto L = [&](int c, int64_t x) { return a.lamE[c * cs + x]; };

@@ -31,6 +31,21 @@ def _soa_view(t):
     return t.permute(1, 2, 3, 0)
 
 
+def plane_pitch(ny, nz):
+    """Cells from one x plane of a device array to the next: ny*nz rounded up
+    to a multiple of 32, so that every x plane of every per-cell array starts
+    on a 128-byte line (a wavefront's 64 consecutive cells then span two lines
+    instead of three). Planes below 4096 cells, where the padding would cost
+    more than 1 % of the memory and alignment does not pay, stay unpadded.
+    FDTD_HIP_PITCH: 0 never pads, 1 pads every plane that is not a multiple of
+    32 cells (as a test of the padded layout on small grids)."""
+    p = int(ny) * int(nz)
+    mode = os.environ.get("FDTD_HIP_PITCH", "auto")
+    if mode == "0" or p % 32 == 0 or (mode != "1" and p < 4096):
+        return p
+    return (p + 31) // 32 * 32
+
+
 def curl_E(E):
     """Forward-difference curl of an E-type (Nx,Ny,Nz,3) array (fdtd/grid.py:29-51).
     Host-side helper for inspection; the time step never calls it."""
@@ -85,11 +100,13 @@ class Grid:
         self.compensated = bool(bd.compensated)
         self.complex = False  # promote_dtypes_to_complex (grid.py:353-356)
         dev = bd.device
-        soa = (3, self.Nx, self.Ny, self.Nz)
-        self._E = torch.zeros(soa, dtype=self.dtype, device=dev)
-        self._H = torch.zeros(soa, dtype=self.dtype, device=dev)
-        self._inv_eps = torch.empty(soa, dtype=self.mat_dtype, device=dev)
-        self._inv_mu = torch.empty(soa, dtype=self.mat_dtype, device=dev)
+        # Device arrays keep x planes `_px` cells apart (plane_pitch); E, H and
+        # the materials are (3, Nx, Ny, Nz) views of (3, Nx, _px) storage.
+        self._px = plane_pitch(self.Ny, self.Nz)
+        self._E = self._alloc_soa(self.dtype)
+        self._H = self._alloc_soa(self.dtype)
+        self._inv_eps = self._alloc_soa(self.mat_dtype)
+        self._inv_mu = self._alloc_soa(self.mat_dtype)
         self._fill_inverse(self._inv_eps, permittivity)
         self._fill_inverse(self._inv_mu, permeability)
         if self.compensated:
@@ -101,9 +118,8 @@ class Grid:
             # a 32-bit word whose byte c holds component c's bits. _E/_H are
             # the (3, Nx, Ny, Nz) float views of the records, _E_lo/_H_lo the
             # words, so that the kernels read a cell with one 16-byte access.
-            rec = (self.Nx, self.Ny, self.Nz, 4)
-            self._E_rec = torch.zeros(rec, dtype=torch.float32, device=dev)
-            self._H_rec = torch.zeros(rec, dtype=torch.float32, device=dev)
+            self._E_rec = self._alloc_cells((4,), torch.float32)
+            self._H_rec = self._alloc_cells((4,), torch.float32)
             self._E = self._E_rec[..., :3].permute(3, 0, 1, 2)
             self._H = self._H_rec[..., :3].permute(3, 0, 1, 2)
             self._E_lo = self._E_rec[..., 3].view(torch.int32)
@@ -125,6 +141,19 @@ class Grid:
         self._plan_key = None
         self._wave = None
         self._wave_key = None
+
+    # ------------------------------------------------------------ storage
+    def _alloc_soa(self, dtype):
+        """Zeroed (3, Nx, Ny, Nz) device array in the pitched layout."""
+        P = self.Ny * self.Nz
+        store = torch.zeros((3, self.Nx, self._px), dtype=dtype, device=bd.device)
+        return store[:, :, :P].unflatten(2, (self.Ny, self.Nz))
+
+    def _alloc_cells(self, trail, dtype):
+        """Zeroed (Nx, Ny, Nz, *trail) device array in the pitched layout."""
+        P = self.Ny * self.Nz
+        store = torch.zeros((self.Nx, self._px) + tuple(trail), dtype=dtype, device=bd.device)
+        return store[:, :P].unflatten(1, (self.Ny, self.Nz))
 
     # ------------------------------------------------------------ materials
     def _fill_inverse(self, dst, value):
@@ -277,7 +306,7 @@ class Grid:
             if v < -n or v >= n:
                 raise IndexError(f"index {v} is out of bounds for a grid axis of size {n}")
             out.append(v + n if v < 0 else v)
-        return (out[0] * self.Ny + out[1]) * self.Nz + out[2]
+        return out[0] * self._px + out[1] * self.Nz + out[2]
 
     def _components(self):
         return list(self.boundaries) + list(self.objects) + list(self.sources) + list(self.detectors)
@@ -314,8 +343,10 @@ class Grid:
         if self.complex:
             return
         cdt = torch.complex128 if self.dtype == torch.float64 else torch.complex64
-        self._E = self._E.to(cdt)
-        self._H = self._H.to(cdt)
+        E, H = self._alloc_soa(cdt), self._alloc_soa(cdt)
+        E.copy_(self._E)
+        H.copy_(self._H)
+        self._E, self._H = E, H
         self.complex = True
         for det in self.detectors:
             det._promote(cdt)
@@ -338,7 +369,9 @@ class Grid:
                 break
             tab = torch.zeros((256, 3), dtype=torch.float64, device=arr.device)
             tab[: uniq.shape[0]] = uniq.to(torch.float64)
-            out += [idx.to(torch.uint8).contiguous(), tab]
+            ids = self._alloc_cells((), torch.uint8)
+            ids.copy_(idx.to(torch.uint8).view(self.Nx, self.Ny, self.Nz))
+            out += [ids, tab]
         self._palette = tuple(out) if out else None
         self._palette_key = key
         return self._palette
@@ -371,7 +404,7 @@ class Grid:
         else:
             dtype_code = _native.F64 if self.dtype == torch.float64 else _native.F32
         plan = _native.Plan(dtype_code, (self.Nx, self.Ny, self.Nz), self.courant_number,
-                            self._E, self._H, self._inv_eps, self._inv_mu)
+                            self._E, self._H, self._inv_eps, self._inv_mu, pitch=self._px)
         if self.compensated:
             plan.set_compensation(self._E_lo, self._H_lo)
         pal = self._material_palette() if self._palette_wanted() else None

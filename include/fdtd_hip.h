@@ -71,6 +71,16 @@ int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number,
 
 int fdtd_plan_destroy(FdtdPlan* plan);
 
+/* Optional, right after fdtd_plan_create: the x-plane pitch px >= ny*nz of
+ * every per-cell array of the plan (fields, materials, records, palette
+ * indices, the adjoint's field-shaped buffers): cell (i, j, k) of a component
+ * is at i*px + j*nz + k and components are nx*px apart (default px = ny*nz,
+ * the reference's contiguous (3, nx, ny, nz) layout, fdtd/grid.py:90-169).
+ * A pitch that rounds a plane up to a multiple of 32 cells starts every x
+ * plane of every array on a 128-byte line; the padding cells are never read
+ * or written. No counterpart in the reference: its arrays are contiguous. */
+int fdtd_plan_set_pitch(FdtdPlan* plan, int64_t px);
+
 /* FDTD_F32C only, required. E and H of such a plan are (nx, ny, nz, 4)
  * float32 records, z fastest: x, y, z of the field at the cell, then a uint32
  * word whose byte c holds component c's compensation term (E_lo / H_lo: the

@@ -23,6 +23,8 @@ the device library gets. Prints one JSON line per case: {"case", "ok",
                       against the reference-autograd fixture, 1e-10
   adjoint_full:<backend>  the transposed H kernel without its store guard
                       against the guarded form, bit for bit
+  adjoint_pitch:<scenario>  a gradient scenario with padded x planes against
+                      unpadded ones, bit for bit
   batch:<seed>        grids of uneven shapes stepped by fdtd_batch_run
                       (direct launches, graph replays and a persistent
                       launch) and by fdtd_batch_run_static, with the x-chunks
@@ -198,6 +200,42 @@ def case_adjoint(name):
     return errs
 
 
+def case_adjoint_pitch(name):
+    """A gradient scenario (record mode) with unpadded x planes and with every
+    plane padded to a multiple of 32 cells (FDTD_HIP_PITCH 0 and 1): the same
+    J, final E and gradient bit for bit."""
+    import adjoint_scenarios as A
+    from fdtd_amd.adjoint import IntensityGradient
+
+    sc = getattr(A, name)
+    out, pitches = [], []
+    keep = os.environ.get("FDTD_HIP_PITCH")
+    try:
+        for pitch in ("0", "1"):
+            os.environ["FDTD_HIP_PITCH"] = pitch
+            fdtd.set_backend("hip.float64")
+            perm = A.design_permittivity(sc)
+            grid, design = A.build(fdtd, sc, torch.from_numpy(perm))
+            prob = IntensityGradient(grid, design, sc["plane"], sc["steps"], checkpoint="record")
+            J, grad = prob.gradient(torch.from_numpy(perm))
+            out.append((J, grad.numpy().copy(), _np(grid.E).copy()))
+            pitches.append(grid._px)
+    finally:
+        if keep is None:
+            os.environ.pop("FDTD_HIP_PITCH", None)
+        else:
+            os.environ["FDTD_HIP_PITCH"] = keep
+    errs = []
+    if pitches[0] == pitches[1]:
+        errs.append(f"the scenario's planes were not padded (pitch {pitches})")
+    if out[0][0] != out[1][0]:
+        errs.append(f"J {out[0][0]!r} != {out[1][0]!r}")
+    for what, a, b in (("gradient", out[0][1], out[1][1]), ("final E", out[0][2], out[1][2])):
+        if not np.array_equal(a, b):
+            errs.append(f"{what} differs with padded planes")
+    return errs
+
+
 def case_adjoint_full(backend):
     """The transposed H kernel without its store guard (tiles covering the
     (y, z) plane) against the guarded form (FDTD_HIP_ADJ_FULL=0): the same
@@ -332,7 +370,7 @@ def case_batch(seed):
 
 CASES = {"fixture": case_fixture, "random": case_random, "random_complex": case_random_complex,
          "compensated": case_compensated, "batch": case_batch, "adjoint": case_adjoint,
-         "adjoint_full": case_adjoint_full, "nonfinite": case_nonfinite}
+         "adjoint_full": case_adjoint_full, "adjoint_pitch": case_adjoint_pitch, "nonfinite": case_nonfinite}
 
 
 def main(argv):

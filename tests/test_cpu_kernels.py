@@ -14,8 +14,10 @@ object), and the guard-free transposed H kernel of tiles that cover the
 nothing else.
 Both tile shapes of the main kernels run: the default (flat tiles for these
 grids, whose nz is not a multiple of 64) and row tiles forced with
-FDTD_HIP_FLAT=0. Any out-of-bounds access aborts the child process with an
-AddressSanitizer report.
+FDTD_HIP_FLAT=0; and the padded plane layout, forced on these small grids with
+FDTD_HIP_PITCH=1 (every x plane whose ny*nz is not a multiple of 32 padded to
+one), must give the same bits. Any out-of-bounds access aborts the child
+process with an AddressSanitizer report.
 """
 
 import json
@@ -36,7 +38,8 @@ def _cases():
     out += [f"random:{s}" for s in range(6)]
     out += [f"random_complex:{s}" for s in range(6)]
     out += ["compensated:g2", "compensated:periodic_x_first", "batch:1", "batch:2", "adjoint:g5", "adjoint:pa",
-            "adjoint_full:hip.float64", "adjoint_full:hip.float32", "nonfinite:g2", "nonfinite:complex_object"]
+            "adjoint_full:hip.float64", "adjoint_full:hip.float32", "adjoint_pitch:TINY_PA", "adjoint_pitch:TINY_ABSORBER",
+            "nonfinite:g2", "nonfinite:complex_object"]
     return out
 
 
@@ -48,12 +51,15 @@ def host_library():
     return build.build(), build.asan_runtime()
 
 
-@pytest.mark.parametrize("flat", ["default", "0"])
-def test_kernels_on_the_host_match_fixtures_and_oracle(host_library, flat):
+@pytest.mark.parametrize("layout", ["default", "row_tiles", "padded_planes"])
+def test_kernels_on_the_host_match_fixtures_and_oracle(host_library, layout):
     lib, asan = host_library
     env = dict(os.environ, LD_PRELOAD=asan, ASAN_OPTIONS="detect_leaks=0:abort_on_error=1")
-    if flat != "default":
-        env["FDTD_HIP_FLAT"] = flat
+    env.pop("FDTD_HIP_PITCH", None)
+    if layout == "row_tiles":
+        env["FDTD_HIP_FLAT"] = "0"
+    elif layout == "padded_planes":
+        env["FDTD_HIP_PITCH"] = "1"
     cases = _cases()
     res = subprocess.run([sys.executable, os.path.join(ROOT, "tests", "cpu_kernels", "replay.py")] + cases,
                          capture_output=True, text=True, env=env, timeout=900)

@@ -245,8 +245,8 @@ def test_transposed_launch_shape_leaves_gradient_unchanged():
         _native.Adjoint.set_launch = orig
     for g in grads:
         assert np.array_equal(g, g0.cpu().numpy())
-    adj = _native.Adjoint(prob._plan(), *prob._coefficients(), torch.zeros_like(prob.grid._E),
-                          torch.zeros_like(prob.grid._H),
+    adj = _native.Adjoint(prob._plan(), *prob._coefficients(), prob.grid._alloc_soa(prob.grid._E.dtype),
+                          prob.grid._alloc_soa(prob.grid._H.dtype),
                           torch.zeros(_native.Adjoint.psi_size(prob._plan()), dtype=prob.grid.dtype, device="cuda"),
                           prob.design_box, prob.box, torch.zeros((3, prob.nd), dtype=torch.float64, device="cuda"))
     with pytest.raises(RuntimeError, match="cx"):

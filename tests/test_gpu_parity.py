@@ -412,10 +412,11 @@ def test_compensated_words_are_the_record_slots():
 
     fdtd.set_backend("hip.float32c")
     grid = fdtd.Grid((6, 5, 4), grid_spacing=50e-9, courant_number=0.5)
-    assert grid._E_rec.shape == (6, 5, 4, 4) and grid._E_rec.is_contiguous()
-    assert grid.E.data_ptr() == grid._E_rec.data_ptr() and grid.E.stride() == (80, 16, 4, 1)
+    assert grid._E_rec.shape == (6, 5, 4, 4) and grid._E_rec.stride() == (grid._px * 4, 16, 4, 1)
+    assert grid.E.data_ptr() == grid._E_rec.data_ptr() and grid.E.stride() == (grid._px * 4, 16, 4, 1)
     assert grid._E_lo.data_ptr() == grid._E_rec.data_ptr() + 12
-    plan = _native.Plan(_native.F32C, (6, 5, 4), grid.courant_number, grid._E, grid._H, grid._inv_eps, grid._inv_mu)
+    plan = _native.Plan(_native.F32C, (6, 5, 4), grid.courant_number, grid._E, grid._H, grid._inv_eps, grid._inv_mu,
+                        pitch=grid._px)
     with pytest.raises(RuntimeError, match="4th float"):
         plan.set_compensation(torch.zeros((6, 5, 4), dtype=torch.int32, device=grid._E.device), grid._H_lo)
     plan.set_compensation(grid._E_lo, grid._H_lo)
