@@ -421,3 +421,40 @@ def test_compensated_material_assignment_keeps_fields_and_precision():
         assert torch.equal(grid.full_precision("E"), e_before)
     finally:
         fdtd.set_backend("hip.float64")
+
+
+def test_padded_plane_layout(monkeypatch):
+    """x planes of 4096 cells and more are padded to a multiple of 32 cells
+    (FDTD_HIP_PITCH: 0 never, 1 every plane); the (Nx, Ny, Nz, 3) views and
+    the cell indices the kernels get follow the padded storage, and writes
+    through the views never touch the padding."""
+    from fdtd_amd.grid import plane_pitch
+
+    monkeypatch.delenv("FDTD_HIP_PITCH", raising=False)
+    assert plane_pitch(250, 250) == 62528 and plane_pitch(256, 256) == 65536
+    assert plane_pitch(18, 14) == 252 and plane_pitch(100, 100) == 10016
+    monkeypatch.setenv("FDTD_HIP_PITCH", "1")
+    assert plane_pitch(18, 14) == 256 and plane_pitch(8, 4) == 32 and plane_pitch(97, 1) == 128
+    monkeypatch.setenv("FDTD_HIP_PITCH", "0")
+    assert plane_pitch(250, 250) == 62500
+    monkeypatch.setenv("FDTD_HIP_PITCH", "1")
+    for backend in ("hip.float64", "hip.float32c"):
+        fdtd.set_backend(backend)
+        g = fdtd.Grid((5, 6, 7), grid_spacing=50e-9, courant_number=0.5)
+        assert g._px == 64
+        assert g._cell(2, 3, 4) == 2 * 64 + 3 * 7 + 4 and g._cell(-1, -1, -1) == 4 * 64 + 5 * 7 + 6
+        rng = np.random.default_rng(0)
+        E = rng.uniform(-1, 1, (5, 6, 7, 3))
+        g.E = torch.from_numpy(E)
+        if g.compensated:
+            store = g._E_rec._base if g._E_rec._base is not None else g._E_rec
+            assert store.shape == (5, 64, 4)
+            assert not store[:, 42:].any()  # the padding of every plane stays zero
+            assert np.abs(_np(g.full_precision("E")) - E).max() <= 2.0 ** -31
+        else:
+            store = g._E._base
+            assert store.shape == (3, 5, 64) and not store[:, :, 42:].any()
+            assert np.array_equal(_np(g.E), E)
+        g.inverse_permittivity = torch.full((5, 6, 7, 3), 0.25, dtype=torch.float64)
+        assert np.all(_np(g.inverse_permittivity) == 0.25)
+    fdtd.set_backend("hip.float64")
