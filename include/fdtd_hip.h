@@ -64,7 +64,9 @@ const char* fdtd_last_error(void);
  * Replaces the state set up by Grid.__init__ (fdtd/grid.py:90-169):
  * E, H (3,nx,ny,nz) read-write (FDTD_F32C: (nx,ny,nz,4) records, see
  * fdtd_plan_set_compensation); inv_eps, inv_mu (3,nx,ny,nz) read-only,
- * read every step so that writes to the grid materials take effect. */
+ * read every step so that writes to the grid materials take effect. The x
+ * planes of these arrays are ny*nz cells apart unless fdtd_plan_set_pitch
+ * says otherwise. */
 int fdtd_plan_create(int dtype, int nx, int ny, int nz, double courant_number,
                      void* E, void* H, const void* inv_eps, const void* inv_mu,
                      FdtdPlan** out);
