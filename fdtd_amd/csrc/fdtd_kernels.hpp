@@ -98,8 +98,8 @@ struct DetDev {
 };
 
 // Cell (i, j, k) of a component is at i*px + j*nz + k: z fastest, x planes
-// `px` cells apart (px >= ny*nz; the host rounds a plane up so that every x
-// plane starts on a cache line, and the cells past ny*nz are never read or
+// `px` cells apart (px >= ny*nz; the host may round a plane up so that every
+// x plane starts on a cache line; the cells past ny*nz are then never read or
 // written), components cs = nx*px apart.
 struct Dims {
   int32_t nx, ny, nz, pad;
