@@ -647,14 +647,15 @@ void batch_free_graph(FdtdBatch* b) {
 }
 
 // Whether a batch deals its x-chunks cyclically to the XCDs: as for one grid,
-// once the fields outgrow the caches (16.7M cells in all). Contiguous runs of
-// work items leave some XCDs working while others wait: the grids' CPML slabs
-// and their unequal shapes make tiles cost unequally. FDTD_HIP_CYCLIC
-// overrides.
+// once the fields outgrow the caches (16.7M cells in all), and when its grids
+// average 1M cells or more. Contiguous runs of large grids leave some XCDs
+// working while others wait (the grids' CPML slabs and unequal shapes make
+// tiles cost unequally); grids of a few MB keep their x-halo planes in the L2
+// of the one XCD that a contiguous run gives them. FDTD_HIP_CYCLIC overrides.
 bool batch_cyclic(const std::vector<FdtdPlan*>& plans) {
   int64_t cells = 0;
   for (const FdtdPlan* p : plans) cells += grid_cells(p);
-  bool cyc = cells >= (int64_t)1 << 24;
+  bool cyc = cells >= (int64_t)1 << 24 && cells / (int64_t)plans.size() >= (int64_t)1 << 20;
   if (const char* env = getenv("FDTD_HIP_CYCLIC")) cyc = atoi(env) != 0;
   return cyc;
 }
